@@ -1,0 +1,387 @@
+// Device functions for the forward light-tracing kernels (gfx950, wave64).
+//
+// Arithmetic follows the reference's CPU forward path in float32, except where a
+// double keeps the same sample: the photon RNG (PCG32 seeded per photon by
+// SplitMix64, src/rng.h:6-42) and the wavelength CDF inversion
+// (src/spectrum.h:374-383) run in integer / double so the GPU draws exactly the
+// same photons as the CPU reference for the same (seedBase, salt).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gp_types.hpp"
+
+namespace gp {
+
+constexpr float kPiF = 3.14159265358979f;
+
+// ------------------------------------------------------------------ vectors
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 normalize(F3 a) {
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ F3 reflectDir(F3 d, F3 n) { return d - n * (2.0f * dot(d, n)); }
+__device__ __forceinline__ F3 load3(const float* p) { return F3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ F3 xyz(float4 v) { return F3{v.x, v.y, v.z}; }
+
+// Orthonormal basis around unit n (Duff et al. 2017, the reference's onb).
+__device__ __forceinline__ void frame(F3 n, F3& t, F3& b) {
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float d = n.x * n.y * a;
+    t = F3{1.0f + sign * n.x * n.x * a, sign * d, -sign * n.x};
+    b = F3{d, sign + n.y * n.y * a, -n.y};
+}
+
+// ------------------------------------------------------------------ rng
+// PCG-XSH-RR 32 (O'Neill). `inc` is not stored per path: it is a pure function of
+// the photon index, recomputed where a kernel draws.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+struct Rng {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    // (next >> 8) * 2^-24: exactly the reference's double uniform, representable in float.
+    __device__ __forceinline__ float uniform() { return (float)(next() >> 8) * 5.9604644775390625e-8f; }
+    __device__ __forceinline__ double uniformD() { return (double)(next() >> 8) * (1.0 / 16777216.0); }
+};
+__device__ __forceinline__ uint64_t photonInc(uint64_t k, uint64_t salt) {
+    return (splitmix64(k ^ salt) << 1u) | 1u;
+}
+__device__ __forceinline__ Rng seedPhoton(uint64_t k, uint64_t salt) {
+    Rng r;
+    r.state = 0;
+    r.inc = photonInc(k, salt);
+    r.next();
+    r.state += splitmix64(k + salt);
+    r.next();
+    return r;
+}
+
+// ------------------------------------------------------------------ spectra
+__device__ __forceinline__ float specEval(const DevSpectrum& s, float w) {
+    if (s.band_hi > s.band_lo && !(w >= s.band_lo && w <= s.band_hi)) return 0.0f;
+    switch (s.kind) {
+        case 0: return s.p[0];
+        case 1: return (w < s.p[2]) ? s.p[0] : s.p[1];
+        case 2: {
+            float t = (w - s.p[2]) / s.p[3];
+            return s.p[0] + s.p[1] * __expf(-0.5f * t * t);
+        }
+        case 4: {
+            float l = w * 1e-3f;
+            float l2 = l * l;
+            float n2 = 1.0f + s.p[0] * l2 / (l2 - s.p[3]) + s.p[1] * l2 / (l2 - s.p[4]) +
+                       s.p[2] * l2 / (l2 - s.p[5]);
+            return sqrtf(n2 > 1.0f ? n2 : 1.0f);
+        }
+        case 5: {
+            float um = w * 1e-3f;
+            return s.p[0] + s.p[1] / (um * um);
+        }
+        default: return 0.0f;
+    }
+}
+
+// CIE 1931 observer, multi-lobe Gaussian fit (Wyman, Sloan, Shirley 2013).
+__device__ __forceinline__ float lobe(float x, float mu, float s1, float s2) {
+    float t = (x - mu) * ((x < mu) ? s1 : s2);
+    return __expf(-0.5f * t * t);
+}
+__device__ __forceinline__ F3 cie(float w) {
+    float X = 0.362f * lobe(w, 442.0f, 0.0624f, 0.0374f) + 1.056f * lobe(w, 599.8f, 0.0264f, 0.0323f) -
+              0.065f * lobe(w, 501.1f, 0.0490f, 0.0382f);
+    float Y = 0.821f * lobe(w, 568.8f, 0.0213f, 0.0247f) + 0.286f * lobe(w, 530.9f, 0.0613f, 0.0322f);
+    float Z = 1.217f * lobe(w, 437.0f, 0.0845f, 0.0278f) + 0.681f * lobe(w, 459.0f, 0.0385f, 0.0725f);
+    return F3{X, Y, Z};
+}
+
+// Inversion of a normalised 1-nm CDF over [360, 830), in double as on the host.
+__device__ __forceinline__ double invertCdf(const double* __restrict__ cdf, double u, double& pdf) {
+    int lo = 0, hi = kLambdaBins;
+    while (lo + 1 < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    double c0 = cdf[lo], c1 = cdf[lo + 1];
+    double frac = (c1 > c0) ? (u - c0) / (c1 - c0) : 0.5;
+    pdf = c1 - c0;
+    return 360.0 + ((double)lo + frac);
+}
+
+// ------------------------------------------------------------------ sampling
+__device__ __forceinline__ F3 cosineDir(F3 n, Rng& rng) {
+    float u1 = rng.uniform();
+    float u2 = rng.uniform();
+    float r = sqrtf(u1);
+    float s, c;
+    sincosf(6.28318530717959f * u2, &s, &c);
+    float lz = sqrtf(1.0f - u1);
+    F3 t, b;
+    frame(n, t, b);
+    return normalize(t * (r * c) + b * (r * s) + n * lz);
+}
+// Power-cosine lobe about mdir (the reference's glossyDirUV).
+__device__ __forceinline__ F3 glossyDir(F3 mdir, float rough, Rng& rng) {
+    float u1 = rng.uniform();
+    float u2 = rng.uniform();
+    float rr = rough < 1e-3f ? 1e-3f : rough;
+    float e = 2.0f / (rr * rr) - 2.0f;
+    if (e < 0.0f) e = 0.0f;
+    float cosT = powf(u1, 1.0f / (e + 1.0f));
+    float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+    float s, c;
+    sincosf(6.28318530717959f * u2, &s, &c);
+    F3 t, b;
+    frame(mdir, t, b);
+    return normalize(t * (sinT * c) + b * (sinT * s) + mdir * cosT);
+}
+
+// ------------------------------------------------------------------ intersection
+struct RayPre {
+    F3 o, d, inv;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+__device__ __forceinline__ float pick(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ RayPre prepareRay(F3 o, F3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.inv = F3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax >= ay && ax >= az) ? 0 : ((ay >= az) ? 1 : 2);
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    float dz = pick(d, kz);
+    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = pick(d, kx) / dz;
+    r.Sy = pick(d, ky) / dz;
+    r.Sz = 1.0f / dz;
+    return r;
+}
+
+// Sign-exact fallback for an edge function that rounded to zero: the two float
+// products are exact in double, so the double difference has the true sign.
+__device__ __forceinline__ float edgeFallback(float a, float b, float c, float d) {
+    double v = (double)a * (double)b - (double)c * (double)d;
+    return v > 0.0 ? 1e-30f : (v < 0.0 ? -1e-30f : 0.0f);
+}
+
+// Watertight two-sided ray/triangle test (Woop, Benthin, Wald 2013) in float.
+// The three edge-function differences are evaluated without FMA contraction so a
+// shared edge is decided identically (with opposite sign) by both triangles.
+__device__ __forceinline__ bool hitTri(const RayPre& r, F3 v0, F3 v1, F3 v2, float tmin, float tmax,
+                                       float& tOut) {
+#pragma clang fp contract(off)
+    F3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = pick(A, r.kz), Bkz = pick(B, r.kz), Ckz = pick(C, r.kz);
+    float Ax = pick(A, r.kx) - r.Sx * Akz, Ay = pick(A, r.ky) - r.Sy * Akz;
+    float Bx = pick(B, r.kx) - r.Sx * Bkz, By = pick(B, r.ky) - r.Sy * Bkz;
+    float Cx = pick(C, r.kx) - r.Sx * Ckz, Cy = pick(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        if (U == 0.0f) U = edgeFallback(Cx, By, Cy, Bx);
+        if (V == 0.0f) V = edgeFallback(Ax, Cy, Ay, Cx);
+        if (W == 0.0f) W = edgeFallback(Bx, Ay, By, Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    float t = T * (1.0f / det);
+    if (!(t >= tmin && t < tmax)) return false;
+    tOut = t;
+    return true;
+}
+
+// Slab test of the two child boxes of a node. Returns entry distances; a child is
+// hit when its entry <= exit. Exit distances are widened by 2^-21 relative so
+// rounding can only over-report a box (Ize 2013, robust BVH traversal).
+struct PairHit {
+    float tl, tr;
+    bool hl, hr;
+};
+__device__ __forceinline__ PairHit slabPair(const RayPre& r, float4 a, float4 b, float4 c, float4 d,
+                                            float tmin, float tmax) {
+    const float widen = 1.00000048f;
+    float lx0 = (a.x - r.o.x) * r.inv.x, lx1 = (b.x - r.o.x) * r.inv.x;
+    float ly0 = (a.y - r.o.y) * r.inv.y, ly1 = (b.y - r.o.y) * r.inv.y;
+    float lz0 = (a.z - r.o.z) * r.inv.z, lz1 = (b.z - r.o.z) * r.inv.z;
+    float rx0 = (c.x - r.o.x) * r.inv.x, rx1 = (d.x - r.o.x) * r.inv.x;
+    float ry0 = (c.y - r.o.y) * r.inv.y, ry1 = (d.y - r.o.y) * r.inv.y;
+    float rz0 = (c.z - r.o.z) * r.inv.z, rz1 = (d.z - r.o.z) * r.inv.z;
+    float lin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
+    float lout = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax)) * widen;
+    float rin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
+    float rout = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax)) * widen;
+    PairHit p;
+    p.tl = lin;
+    p.tr = rin;
+    p.hl = lin <= lout;
+    p.hr = rin <= rout;
+    return p;
+}
+
+// Traversal stack: the top `SD` entries live in LDS at [slot * stride + lane], deeper
+// entries spill to a per-thread global area. The host guarantees SD + spill depth
+// >= the tree's depth, and each level pushes at most one entry, so it never overflows.
+template <int SD>
+struct Stack {
+    int* sidx;            // LDS, SD * stride ints
+    float* st;            // LDS, SD * stride floats
+    int stride;
+    int2* spill;          // global, spillDepth entries for this thread (may be null)
+    int sp;
+    __device__ __forceinline__ void push(int node, float t) {
+        if (sp < SD) {
+            sidx[sp * stride] = node;
+            st[sp * stride] = t;
+        } else {
+            spill[sp - SD] = make_int2(node, __float_as_int(t));
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(int& node, float& t) {
+        --sp;
+        if (sp < SD) {
+            node = sidx[sp * stride];
+            t = st[sp * stride];
+        } else {
+            int2 v = spill[sp - SD];
+            node = v.x;
+            t = __int_as_float(v.y);
+        }
+    }
+};
+
+// Closest hit. Returns the leaf-order triangle index (or -1) and its distance.
+template <int SD>
+__device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
+                                            const float4* __restrict__ tris, const RayPre& r,
+                                            float tmin, float& tmax, Stack<SD>& stk) {
+    int best = -1;
+    int node = 0;
+    stk.sp = 0;
+    while (true) {
+        if (node >= 0) {
+            const float4* nd = nodes + 4 * node;
+            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+            PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
+            int left = __float_as_int(a.w), right = __float_as_int(b.w);
+            if (ph.hl && ph.hr) {
+                bool lfirst = ph.tl <= ph.tr;
+                stk.push(lfirst ? right : left, lfirst ? ph.tr : ph.tl);
+                node = lfirst ? left : right;
+                continue;
+            }
+            if (ph.hl) { node = left; continue; }
+            if (ph.hr) { node = right; continue; }
+        } else {
+            int code = ~node;
+            int first = code >> 3, count = (code & 7) + 1;
+            for (int k = 0; k < count; ++k) {
+                const float4* tp = tris + 3 * (first + k);
+                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                float t;
+                if (hitTri(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, t)) {
+                    tmax = t;
+                    best = first + k;
+                }
+            }
+        }
+        // pop the next entry that can still hold a closer hit
+        bool found = false;
+        while (stk.sp > 0) {
+            float t;
+            stk.pop(node, t);
+            if (t <= tmax) { found = true; break; }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// Any hit on [tmin, tmax).
+template <int SD>
+__device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
+                                         const float4* __restrict__ tris, const RayPre& r,
+                                         float tmin, float tmax, Stack<SD>& stk) {
+    int node = 0;
+    stk.sp = 0;
+    while (true) {
+        if (node >= 0) {
+            const float4* nd = nodes + 4 * node;
+            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+            PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
+            int left = __float_as_int(a.w), right = __float_as_int(b.w);
+            if (ph.hl && ph.hr) {
+                bool lfirst = ph.tl <= ph.tr;
+                stk.push(lfirst ? right : left, 0.0f);
+                node = lfirst ? left : right;
+                continue;
+            }
+            if (ph.hl) { node = left; continue; }
+            if (ph.hr) { node = right; continue; }
+        } else {
+            int code = ~node;
+            int first = code >> 3, count = (code & 7) + 1;
+            for (int k = 0; k < count; ++k) {
+                const float4* tp = tris + 3 * (first + k);
+                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                float t;
+                if (hitTri(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, t)) return true;
+            }
+        }
+        if (stk.sp == 0) break;
+        float t;
+        stk.pop(node, t);
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ unsigned laneId() { return __lane_id(); }
+// Wave-aggregated append: every lane with `pred` gets a unique slot from *counter.
+__device__ __forceinline__ unsigned waveAppend(unsigned* counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    unsigned cnt = __popcll(mask);
+    unsigned base = 0;
+    int leader = __ffsll((long long)mask) - 1;
+    if (cnt) {
+        if ((int)laneId() == leader) base = atomicAdd(counter, cnt);
+        base = __shfl(base, leader < 0 ? 0 : leader);
+    }
+    unsigned lo = (unsigned)(mask & 0xffffffffull), hi = (unsigned)(mask >> 32);
+    unsigned rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    return base + rank;
+}
+
+}  // namespace gp

@@ -1,0 +1,593 @@
+// Wavefront forward light-tracing kernels for CDNA4 (gfx950).
+//
+// One iteration of the wavefront:
+//   k_schedule  (1 block)  refill free queue slots from the jobs' photon budgets
+//   k_emit                 emission + hero bundle + direct light->camera connection
+//   k_extend               closest hit for every live path (LDS-stack traversal)
+//   k_shade                material scatter, Russian roulette, camera connection,
+//                          survivors compacted into the other queue by wave ballot
+//   k_connect              any-hit for every connection; unoccluded ones splat into
+//                          the int64 fixed-point film with 64-bit atomics
+// All kernels are fixed-grid and grid-stride over counts that live on the device,
+// so an iteration needs no host round trip and can be captured in a hipGraph.
+//
+// Per path the arithmetic follows the reference's Renderer::tracePhotonHero /
+// tracePhotonHeroLoop (src/render.h:2252-2631), connectGeom / connectHero
+// (:499-577) and refractOrReflect (:2657-2694), in float32.
+#include <hip/hip_runtime.h>
+
+#include "gp_device.hpp"
+#include "gp_kernels.hpp"
+
+namespace gp {
+
+namespace {
+
+constexpr float kInvPi = 0.318309886183791f;
+
+__device__ __forceinline__ unsigned long long toFixed(double v, double scale) {
+    double q = v * scale;
+    return q > 0.0 ? (unsigned long long)__double2ll_rn(q) : 0ull;
+}
+
+__device__ __forceinline__ unsigned long long shflXor64(unsigned long long v, int m) {
+    unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    lo = __shfl_xor(lo, m);
+    hi = __shfl_xor(hi, m);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Per-lane accumulators for one job at a time; flushed with a segmented wave
+// reduction (one atomic per distinct job in the wave and quantity).
+template <int K>
+struct JobAcc {
+    int job = -1;
+    unsigned long long v[K] = {};
+};
+
+template <int K, class Addr>
+__device__ __forceinline__ void flushLane(JobAcc<K>& a, Addr addr) {
+    if (a.job >= 0)
+        for (int k = 0; k < K; ++k)
+            if (a.v[k]) atomicAdd(addr(a.job, k), a.v[k]);
+    for (int k = 0; k < K; ++k) a.v[k] = 0;
+}
+
+template <int K, class Addr>
+__device__ __forceinline__ void accumulate(JobAcc<K>& a, int job, Addr addr) {
+    if (a.job != job) {
+        flushLane(a, addr);
+        a.job = job;
+    }
+}
+
+// Called by every lane of the wave (converged).
+template <int K, class Addr>
+__device__ __forceinline__ void flushWave(JobAcc<K>& a, Addr addr) {
+    unsigned long long remaining = __ballot(a.job >= 0);
+    const unsigned lane = __lane_id();
+    while (remaining) {
+        int leader = __ffsll((long long)remaining) - 1;
+        int lj = __shfl(a.job, leader);
+        bool mine = (a.job == lj) && ((remaining >> lane) & 1ull);
+        unsigned long long mask = __ballot(mine);
+        for (int k = 0; k < K; ++k) {
+            unsigned long long s = mine ? a.v[k] : 0ull;
+            for (int m = 32; m >= 1; m >>= 1) s += shflXor64(s, m);
+            if ((int)lane == leader && s) atomicAdd(addr(lj, k), s);
+        }
+        remaining &= ~mask;
+    }
+}
+
+struct EnergyAddr {
+    const DevJob* jobs;
+    __device__ unsigned long long* operator()(int j, int k) const {
+        return (unsigned long long*)(jobs[j].energy + k);
+    }
+};
+struct CounterAddr {
+    const DevJob* jobs;
+    int base;
+    __device__ unsigned long long* operator()(int j, int k) const {
+        return jobs[j].counters + base + k;
+    }
+};
+
+// Camera connection record for a diffuse (or emitter) vertex at p with shading
+// normal n (flat triangles: n is also the oriented geometric normal, so the
+// reference's terminator factor and adjoint correction are exactly 1).
+__device__ __forceinline__ bool makeConnection(const DevJob& J, int job, F3 p, F3 n, const float* lam,
+                                               const float* beta, const float* rho, int nUp,
+                                               float4& co, float4& cd, float4& cx) {
+    const F3 eye = load3(J.eye);
+    F3 toCam = eye - p;
+    float dist = sqrtf(dot(toCam, toCam));
+    F3 wdir = toCam * (1.0f / dist);
+    float cosSurf = dot(n, wdir);
+    if (!(cosSurf > 0.0f)) return false;
+    F3 dd = p - eye;
+    float cz = dot(dd, load3(J.w));
+    if (!(cz > 1e-9f)) return false;
+    float ix = (dot(dd, load3(J.u)) / cz) / J.tanHalfX;
+    float iy = (dot(dd, load3(J.v)) / cz) / J.tanHalfY;
+    if (!(ix >= -1.0f && ix < 1.0f && iy >= -1.0f && iy < 1.0f)) return false;
+    int px = (int)((ix * 0.5f + 0.5f) * (float)J.resX);
+    int py = (int)((iy * 0.5f + 0.5f) * (float)J.resY);
+    px = min(max(px, 0), J.resX - 1);
+    py = min(max(py, 0), J.resY - 1);
+    float dist2 = dot(dd, dd);
+    float cosCam = cz / sqrtf(dist2);
+    float denom = dist2 * (J.pixelArea * cosCam * cosCam * cosCam);
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    for (int i = 0; i < nUp; ++i) {
+        float c = beta[i] * (rho[i] * kInvPi) * cosSurf / denom;
+        F3 w = cie(lam[i]);
+        X += w.x * c;
+        Y += w.y * c;
+        Z += w.z * c;
+    }
+    F3 org = p + n * kRayEps;
+    co = make_float4(org.x, org.y, org.z, (dist - 2.0f * kRayEps) - kRayEps);
+    cd = make_float4(wdir.x, wdir.y, wdir.z, __int_as_float(py * J.resX + px));
+    cx = make_float4(X, Y, Z, __int_as_float(job | (nUp << 16)));
+    return true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ schedule
+__global__ void k_schedule(Queues q, int cur) {
+    if (threadIdx.x != 0) return;
+    Ctrl* c = q.ctrl;
+    const unsigned it = c->iter;
+    const unsigned alive = c->nPaths[cur];
+    unsigned freeSlots = c->capacity > alive ? c->capacity - alive : 0u;
+    unsigned total = 0;
+    const int n = c->nOrder;
+    for (int k = 0; k < n; ++k) {
+        const int j = c->order[k];
+        c->emitPrefix[k] = total;
+        c->emitJob[k] = j;
+        c->emitStart[k] = c->nextPhoton[j];
+        unsigned long long rem = (c->active[j] && c->endPhoton[j] > c->nextPhoton[j])
+                                     ? c->endPhoton[j] - c->nextPhoton[j] : 0ull;
+        unsigned room = freeSlots - total;
+        unsigned take = rem < (unsigned long long)room ? (unsigned)rem : room;
+        if (take) {
+            c->nextPhoton[j] += take;
+            if (c->nextPhoton[j] == c->endPhoton[j]) c->lastEmitIter[j] = (int)it;
+        }
+        total += take;
+    }
+    c->emitPrefix[n] = total;
+    c->nEmit = total;
+    c->emitBase = alive;
+    c->nPaths[cur] = alive + total;
+    c->nPaths[cur ^ 1] = 0;
+    c->nConn = 0;
+    c->iter = it + 1;
+}
+
+// ------------------------------------------------------------------ emit
+__global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
+    const Ctrl* c = q.ctrl;
+    const unsigned nEmit = c->nEmit;
+    const unsigned base = c->emitBase;
+    const int nOrder = c->nOrder;
+    JobAcc<1> eacc;   // emitted
+    JobAcc<1> pacc;   // photons
+    EnergyAddr ea{q.jobs};
+    CounterAddr pa{q.jobs, 0};
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nEmit; s += stride) {
+        int k = 0;
+        if (nOrder > 1) {
+            int lo = 0, hi = nOrder;   // largest k with emitPrefix[k] <= s
+            while (lo + 1 < hi) {
+                int mid = (lo + hi) >> 1;
+                if (c->emitPrefix[mid] <= s) lo = mid;
+                else hi = mid;
+            }
+            k = lo;
+        }
+        const int j = c->emitJob[k];
+        const DevJob& J = q.jobs[j];
+        const unsigned long long idx = c->emitStart[k] + (s - c->emitPrefix[k]);
+        Rng rng = seedPhoton(J.seedBase + idx, J.salt);
+        int ei = 0;
+        if (J.nEm > 1) {
+            double u = rng.uniformD();
+            int lo = 0, hi = J.nEm - 1;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (J.emCdf[mid] < u) lo = mid + 1;
+                else hi = mid;
+            }
+            ei = lo;
+        }
+        const DevEmitter& E = J.ems[ei];
+        float u1 = rng.uniform();
+        float u2 = rng.uniform();
+        F3 org = load3(E.origin) + load3(E.u) * u1 + load3(E.v) * u2;
+        F3 en = load3(E.normal);
+        F3 dir = E.collimated ? load3(E.beam) : cosineDir(en, rng);
+        const int C = J.heroC;
+        double ub = rng.uniformD();
+        float lam[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
+        float beta[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
+        double pdf0;
+        lam[0] = (float)invertCdf(E.cdf, ub, pdf0);
+        const bool dead = !(pdf0 > 0.0);
+        for (int i = 1; i < C; ++i) {
+            double uu = ub + (double)i / (double)C;
+            if (uu >= 1.0) uu -= 1.0;
+            double pd;
+            lam[i] = (float)invertCdf(E.cdf, uu, pd);
+        }
+        const float b = J.basePower / (float)C;
+        for (int i = 0; i < C; ++i) beta[i] = b;
+        accumulate(eacc, j, ea);
+        accumulate(pacc, j, pa);
+        pacc.v[0] += 1;
+        float4 co, cd, cx;
+        bool conn = false;
+        if (!dead) {
+            eacc.v[0] += toFixed((double)b * (double)C, J.energyScale);
+            const float one[kMaxHero] = {1.f, 1.f, 1.f, 1.f};
+            conn = makeConnection(J, j, org, en, lam, beta, one, C, co, cd, cx);
+        }
+        const unsigned ci = waveAppend(&q.ctrl->nConn, conn);
+        if (conn) {
+            q.co[ci] = co;
+            q.cd[ci] = cd;
+            q.cx[ci] = cx;
+        }
+        F3 o = org + dir * kRayEps;
+        const unsigned slot = base + s;
+        q.ro[cur][slot] = make_float4(o.x, o.y, o.z, 0.f);
+        q.rd[cur][slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
+        q.lam[cur][slot] = make_float4(lam[0], lam[1], lam[2], lam[3]);
+        q.beta[cur][slot] = make_float4(beta[0], beta[1], beta[2], beta[3]);
+        unsigned meta = packMeta((unsigned)j, 0u, (unsigned)C) | (dead ? 0x80000000u : 0u);
+        q.st[cur][slot] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), (unsigned)idx, meta);
+    }
+    flushWave(eacc, ea);
+    flushWave(pacc, pa);
+}
+
+// ------------------------------------------------------------------ extend
+template <int SD>
+__global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
+    __shared__ int sIdx[SD * 256];
+    __shared__ float sT[SD * 256];
+    const unsigned n = q.ctrl->nPaths[cur];
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
+                  q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    JobAcc<1> sacc;
+    CounterAddr sa{q.jobs, 1};
+    for (unsigned i = gtid; i < n; i += stride) {
+        const uint4 s4 = q.st[cur][i];
+        int best = -1;
+        float t = INFINITY;
+        if (!(s4.w & 0x80000000u)) {
+            const int j = (int)(s4.w & 0xffffu);
+            const DevJob& J = q.jobs[j];
+            const float4 o = q.ro[cur][i];
+            const float4 d = q.rd[cur][i];
+            RayPre r = prepareRay(xyz(o), xyz(d));
+            float tmax = INFINITY;
+            best = traceClosest<SD>(J.nodes, J.tris, r, kRayEps, tmax, stk);
+            t = tmax;
+            accumulate(sacc, j, sa);
+            sacc.v[0] += 1;
+        }
+        q.hit[i] = make_float2(t, __int_as_float(best));
+    }
+    flushWave(sacc, sa);
+}
+
+// ------------------------------------------------------------------ shade
+__global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
+    const int nxt = cur ^ 1;
+    const unsigned n = q.ctrl->nPaths[cur];
+    const unsigned stride = gridDim.x * blockDim.x;
+    JobAcc<3> eacc;   // absorbed, escaped, residual -> energy[1], [3], [4]
+    EnergyAddr ea{q.jobs};
+    auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4 s4 = q.st[cur][i];
+        bool alive = false, conn = false;
+        float4 co, cd, cx;
+        F3 no, nd;
+        float lam[kMaxHero], beta[kMaxHero];
+        unsigned meta = s4.w;
+        Rng rng;
+        rng.state = (uint64_t)s4.x | ((uint64_t)s4.y << 32);
+        if (!(meta & 0x80000000u)) {
+            const int j = (int)(meta & 0xffffu);
+            const int bounce = (int)((meta >> 16) & 0xffu);
+            int nUp = (int)((meta >> 24) & 7u);
+            const DevJob& J = q.jobs[j];
+            accumulate(eacc, j, eaddr);
+            const float4 l4 = q.lam[cur][i], b4 = q.beta[cur][i];
+            lam[0] = l4.x; lam[1] = l4.y; lam[2] = l4.z; lam[3] = l4.w;
+            beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z; beta[3] = b4.w;
+            auto activeSum = [&]() {
+                double s = 0.0;
+                for (int k = 0; k < nUp; ++k) s += (double)beta[k];
+                return s;
+            };
+            const float2 h = q.hit[i];
+            const int tri = __float_as_int(h.y);
+            if (tri < 0) {
+                eacc.v[1] += toFixed(activeSum(), J.energyScale);
+            } else {
+                const float4 o4 = q.ro[cur][i], d4 = q.rd[cur][i];
+                const F3 d = xyz(d4);
+                const F3 p = xyz(o4) + d * h.x;
+                const float4 tn = J.trin[tri];
+                const F3 gn = xyz(tn);
+                const DevMaterial& M = J.mats[__float_as_int(tn.w)];
+                const F3 n = (dot(d, gn) < 0.0f) ? gn : -gn;
+                rng.inc = photonInc(J.seedBase + s4.z, J.salt);
+                const bool roulette = bounce >= J.rrDepth;
+                alive = true;
+                if (M.type == 1) {
+                    // Dielectric: de-hero, then Fresnel reflect or Snell refract (exterior
+                    // index 1). The side is read off the geometric normal (entering when the
+                    // ray runs against it), as the reference does.
+                    if (nUp > 1) {
+                        beta[0] *= (float)J.heroC;
+                        nUp = 1;
+                    }
+                    const float eta_g = specEval(M.ior, lam[0]);
+                    const bool entering = dot(d, gn) < 0.0f;
+                    const F3 nl = entering ? gn : -gn;
+                    const float n1 = entering ? 1.0f : eta_g;
+                    const float n2 = entering ? eta_g : 1.0f;
+                    const float eta = n1 / n2;
+                    const float cosI = -dot(d, nl);
+                    const float sin2t = eta * eta * (1.0f - cosI * cosI);
+                    F3 out;
+                    bool refracted = false;
+                    if (sin2t > 1.0f) {
+                        out = reflectDir(d, nl);
+                    } else {
+                        const float cosT = sqrtf(1.0f - sin2t);
+                        const float rs = (n1 * cosI - n2 * cosT) / (n1 * cosI + n2 * cosT);
+                        const float rp = (n1 * cosT - n2 * cosI) / (n1 * cosT + n2 * cosI);
+                        const float R = 0.5f * (rs * rs + rp * rp);
+                        if (rng.uniform() < R) {
+                            out = reflectDir(d, nl);
+                        } else {
+                            out = d * eta + nl * (eta * cosI - cosT);
+                            refracted = true;
+                        }
+                    }
+                    out = normalize(out);
+                    if (M.roughness > 1e-3f) {
+                        F3 pert = glossyDir(out, M.roughness, rng);
+                        bool ok = refracted ? (dot(pert, nl) < 0.0f) : (dot(pert, nl) > 0.0f);
+                        if (ok) out = pert;
+                    }
+                    no = p + out * kRayEps;
+                    nd = out;
+                } else {
+                    float c[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
+                    float q0 = 0.0f;
+                    for (int k = 0; k < nUp; ++k) {
+                        float v = specEval(M.reflect, lam[k]);
+                        c[k] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                        q0 = (k == 0 || c[k] > q0) ? c[k] : q0;
+                    }
+                    const bool diffuse = (M.type == 0);
+                    if (diffuse) conn = makeConnection(J, j, p, n, lam, beta, c, nUp, co, cd, cx);
+                    if (roulette) {
+                        if (rng.uniform() >= q0) {
+                            eacc.v[0] += toFixed(activeSum(), J.energyScale);
+                            alive = false;
+                        } else {
+                            double ab = 0.0;
+                            for (int k = 0; k < nUp; ++k) {
+                                float w = c[k] / q0;
+                                ab += (double)beta[k] * (double)(1.0f - w);
+                                beta[k] *= w;
+                            }
+                            eacc.v[0] += toFixed(ab, J.energyScale);
+                        }
+                    } else {
+                        if (!(q0 > 0.0f)) {
+                            eacc.v[0] += toFixed(activeSum(), J.energyScale);
+                            alive = false;
+                        } else {
+                            double ab = 0.0;
+                            for (int k = 0; k < nUp; ++k) {
+                                ab += (double)beta[k] * (double)(1.0f - c[k]);
+                                beta[k] *= c[k];
+                            }
+                            eacc.v[0] += toFixed(ab, J.energyScale);
+                        }
+                    }
+                    if (alive) {
+                        F3 wo;
+                        if (diffuse) {
+                            wo = cosineDir(n, rng);
+                        } else if (M.type == 2) {
+                            wo = reflectDir(d, n);
+                        } else {
+                            wo = glossyDir(reflectDir(d, n), M.roughness, rng);
+                            if (dot(wo, n) <= 0.0f) {
+                                eacc.v[0] += toFixed(activeSum(), J.energyScale);
+                                alive = false;
+                            }
+                        }
+                        no = p + n * kRayEps;
+                        nd = wo;
+                    }
+                }
+                if (alive && bounce + 1 >= J.maxDepth) {
+                    eacc.v[2] += toFixed(activeSum(), J.energyScale);
+                    alive = false;
+                }
+                meta = packMeta((unsigned)j, (unsigned)(bounce + 1), (unsigned)nUp);
+            }
+        }
+        const unsigned ci = waveAppend(&q.ctrl->nConn, conn);
+        if (conn) {
+            q.co[ci] = co;
+            q.cd[ci] = cd;
+            q.cx[ci] = cx;
+        }
+        const unsigned pi = waveAppend(&q.ctrl->nPaths[nxt], alive);
+        if (alive) {
+            q.ro[nxt][pi] = make_float4(no.x, no.y, no.z, 0.f);
+            q.rd[nxt][pi] = make_float4(nd.x, nd.y, nd.z, 0.f);
+            q.lam[nxt][pi] = make_float4(lam[0], lam[1], lam[2], lam[3]);
+            q.beta[nxt][pi] = make_float4(beta[0], beta[1], beta[2], beta[3]);
+            q.st[nxt][pi] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), s4.z, meta);
+        }
+    }
+    flushWave(eacc, eaddr);
+}
+
+// ------------------------------------------------------------------ connect
+template <int SD>
+__global__ void __launch_bounds__(256) k_connect(Queues q) {
+    __shared__ int sIdx[SD * 256];
+    __shared__ float sT[SD * 256];
+    const unsigned n = q.ctrl->nConn;
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
+                  q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    JobAcc<2> cacc;   // connections, splats -> counters[2], [3]
+    CounterAddr ca{q.jobs, 2};
+    for (unsigned i = gtid; i < n; i += stride) {
+        const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
+        const unsigned tag = (unsigned)__float_as_int(cx.w);
+        const int j = (int)(tag & 0xffffu);
+        const DevJob& J = q.jobs[j];
+        accumulate(cacc, j, ca);
+        cacc.v[0] += 1;
+        RayPre r = prepareRay(xyz(co), xyz(cd));
+        const bool occ = traceAny<SD>(J.nodes, J.tris, r, kRayEps, co.w, stk);
+        if (!occ) {
+            cacc.v[1] += 1;
+            const int pix = __float_as_int(cd.w);
+            unsigned long long* f = (unsigned long long*)(J.film + 4 * (size_t)pix);
+            const unsigned long long X = min(toFixed((double)cx.x, J.filmScale), J.filmQmax);
+            const unsigned long long Y = min(toFixed((double)cx.y, J.filmScale), J.filmQmax);
+            const unsigned long long Z = min(toFixed((double)cx.z, J.filmScale), J.filmQmax);
+            if (X) atomicAdd(f + 0, X);
+            if (Y) atomicAdd(f + 1, Y);
+            if (Z) atomicAdd(f + 2, Z);
+            atomicAdd(f + 3, (unsigned long long)(tag >> 16));
+        }
+    }
+    flushWave(cacc, ca);
+}
+
+// ------------------------------------------------------------------ probes
+template <int SD>
+__global__ void __launch_bounds__(256) k_probe_closest(const float4* nodes, const float4* tris,
+                                                      const float* rays, long long n, float tmin,
+                                                      int* prim, float* tout, int2* spill,
+                                                      int spillDepth) {
+    __shared__ int sIdx[SD * 256];
+    __shared__ float sT[SD * 256];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
+                  spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
+    for (long long i = gtid; i < n; i += stride) {
+        const float* rp = rays + 6 * i;
+        RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
+        float tmax = INFINITY;
+        int best = traceClosest<SD>(nodes, tris, r, tmin, tmax, stk);
+        prim[i] = best >= 0 ? __float_as_int(tris[3 * best].w) : -1;
+        tout[i] = best >= 0 ? tmax : INFINITY;
+    }
+}
+
+template <int SD>
+__global__ void __launch_bounds__(256) k_probe_any(const float4* nodes, const float4* tris,
+                                                  const float* rays, const float* tmax, long long n,
+                                                  float tmin, unsigned char* occ, int2* spill,
+                                                  int spillDepth) {
+    __shared__ int sIdx[SD * 256];
+    __shared__ float sT[SD * 256];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
+                  spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
+    for (long long i = gtid; i < n; i += stride) {
+        const float* rp = rays + 6 * i;
+        RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
+        occ[i] = traceAny<SD>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
+    }
+}
+
+__global__ void k_fill64(unsigned long long* p, size_t n, unsigned long long v) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ------------------------------------------------------------------ launchers
+void launchSchedule(const Queues& q, int cur, hipStream_t s) {
+    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, s, q, cur);
+}
+void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
+}
+void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
+}
+void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
+    switch (q.lds) {
+        case 8: hipLaunchKernelGGL(k_extend<8>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
+        case 24: hipLaunchKernelGGL(k_extend<24>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
+        case 32: hipLaunchKernelGGL(k_extend<32>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
+        default: hipLaunchKernelGGL(k_extend<16>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
+    }
+}
+void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
+    switch (q.lds) {
+        case 8: hipLaunchKernelGGL(k_connect<8>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
+        case 24: hipLaunchKernelGGL(k_connect<24>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
+        case 32: hipLaunchKernelGGL(k_connect<32>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
+        default: hipLaunchKernelGGL(k_connect<16>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
+    }
+}
+void launchProbeClosest(const float4* nodes, const float4* tris, const float* rays, long long n,
+                        float tmin, int* prim, float* t, int2* spill, int spillDepth, int lds,
+                        int blocks, int threads, hipStream_t s) {
+    (void)threads;
+    switch (lds) {
+        case 8: hipLaunchKernelGGL(k_probe_closest<8>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
+        case 24: hipLaunchKernelGGL(k_probe_closest<24>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
+        case 32: hipLaunchKernelGGL(k_probe_closest<32>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
+        default: hipLaunchKernelGGL(k_probe_closest<16>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
+    }
+}
+void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, const float* tmax,
+                    long long n, float tmin, unsigned char* occ, int2* spill, int spillDepth,
+                    int lds, int blocks, int threads, hipStream_t s) {
+    (void)threads;
+    switch (lds) {
+        case 8: hipLaunchKernelGGL(k_probe_any<8>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
+        case 24: hipLaunchKernelGGL(k_probe_any<24>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
+        case 32: hipLaunchKernelGGL(k_probe_any<32>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
+        default: hipLaunchKernelGGL(k_probe_any<16>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
+    }
+}
+void launchFill64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill64, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace gp

@@ -1,0 +1,50 @@
+// Host-side launch interface of the wavefront kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gp_types.hpp"
+
+namespace gp {
+
+// Device pointers of one wavefront's queues. Path queues are double-buffered:
+// iteration k reads queue (k & 1) and writes survivors to the other.
+struct Queues {
+    float4* ro[2];
+    float4* rd[2];
+    float4* lam[2];
+    float4* beta[2];
+    uint4* st[2];
+    float2* hit;
+    float4* co;      // connection origin.xyz, segment end
+    float4* cd;      // connection dir.xyz, pixel index
+    float4* cx;      // connection X, Y, Z, (job | nUp << 16)
+    Ctrl* ctrl;
+    const DevJob* jobs;
+    int2* spill;     // traversal stack spill, spillDepth entries per thread
+    int spillDepth;
+    int lds;         // LDS stack entries per lane (8, 16, 24 or 32)
+};
+
+struct LaunchShape {
+    int blocksExtend, threadsExtend;
+    int blocksShade, threadsShade;
+};
+
+void launchSchedule(const Queues& q, int cur, hipStream_t s);
+void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
+void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
+void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
+void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s);
+
+// Batch probes over host-provided rays (already on the device).
+void launchProbeClosest(const float4* nodes, const float4* tris, const float* rays, long long n,
+                        float tmin, int* prim, float* t, int2* spill, int spillDepth, int lds,
+                        int blocks, int threads, hipStream_t s);
+void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, const float* tmax,
+                    long long n, float tmin, unsigned char* occ, int2* spill, int spillDepth,
+                    int lds, int blocks, int threads, hipStream_t s);
+
+// Zero-fill of 64-bit words by a kernel (usable inside captured graphs).
+void launchFill64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
+
+}  // namespace gp

@@ -1,0 +1,465 @@
+// Host runtime: device discovery, camera basis, film scaling, wavefront engine.
+#include "gp_runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace gp {
+
+namespace {
+thread_local std::string g_err;
+}
+
+gp_status fail(gp_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+const char* lastError() { return g_err.c_str(); }
+
+// ------------------------------------------------------------------ device
+DeviceState& device() {
+    static DeviceState st = [] {
+        DeviceState d;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            d.why = "no HIP device";
+            return d;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            d.why = "hipGetDeviceProperties failed";
+            return d;
+        }
+        d.name = prop.name;
+        d.arch = prop.gcnArchName;
+        d.cus = prop.multiProcessorCount;
+        d.ldsPerBlock = prop.sharedMemPerBlock;
+        if (d.arch.rfind("gfx950", 0) != 0) {
+            d.why = "device arch " + d.arch + " is not gfx950";
+            return d;
+        }
+        if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&d.stream) != hipSuccess) {
+            d.why = "stream creation failed";
+            return d;
+        }
+        d.ok = true;
+        d.status = GP_OK;
+        return d;
+    }();
+    return st;
+}
+
+gp_status requireDevice() {
+    DeviceState& d = device();
+    if (!d.ok) return fail(GP_ERR_NO_DEVICE, d.why);
+    return GP_OK;
+}
+
+// ------------------------------------------------------------------ camera
+static void sub3(const double* a, const double* b, double* o) { for (int i = 0; i < 3; ++i) o[i] = a[i] - b[i]; }
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void unit3(double* a) {
+    double l = std::sqrt(dot3(a, a));
+    for (int i = 0; i < 3; ++i) a[i] = a[i] / l;
+}
+
+bool cameraBasis(const gp_camera_desc& c, CamBasis& o) {
+    if (c.res_x <= 0 || c.res_y <= 0 || !(c.fov_y_deg > 0.0 && c.fov_y_deg < 180.0)) return false;
+    for (int i = 0; i < 3; ++i) o.eye[i] = c.eye[i];
+    sub3(c.target, c.eye, o.w);
+    unit3(o.w);
+    cross3(o.w, c.up, o.u);
+    unit3(o.u);
+    cross3(o.u, o.w, o.v);
+    o.tanHalfY = std::tan(0.5 * c.fov_y_deg * 3.141592653589793 / 180.0);
+    o.tanHalfX = o.tanHalfY * (double)c.res_x / (double)c.res_y;
+    o.pixelArea = (4.0 * o.tanHalfX * o.tanHalfY) / ((double)c.res_x * (double)c.res_y);
+    o.resX = c.res_x;
+    o.resY = c.res_y;
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(o.u[i]) || !std::isfinite(o.v[i]) || !std::isfinite(o.w[i])) return false;
+    return true;
+}
+
+// ------------------------------------------------------------------ distance bound
+// Closest point on a triangle (Ericson, Real-Time Collision Detection 5.1.5).
+static double pointTriDist2(const double* p, const double* a, const double* b, const double* c) {
+    double ab[3], ac[3], ap[3];
+    sub3(b, a, ab); sub3(c, a, ac); sub3(p, a, ap);
+    double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    double q[3];
+    auto dist2 = [&](const double* x) { double d[3]; sub3(p, x, d); return dot3(d, d); };
+    if (d1 <= 0 && d2 <= 0) return dist2(a);
+    double bp[3]; sub3(p, b, bp);
+    double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return dist2(b);
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        double v = d1 / (d1 - d3);
+        for (int i = 0; i < 3; ++i) q[i] = a[i] + v * ab[i];
+        return dist2(q);
+    }
+    double cp[3]; sub3(p, c, cp);
+    double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return dist2(c);
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        double w = d2 / (d2 - d6);
+        for (int i = 0; i < 3; ++i) q[i] = a[i] + w * ac[i];
+        return dist2(q);
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int i = 0; i < 3; ++i) q[i] = b[i] + w * (c[i] - b[i]);
+        return dist2(q);
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    for (int i = 0; i < 3; ++i) q[i] = a[i] + ab[i] * v + ac[i] * w;
+    return dist2(q);
+}
+static double pointBoxDist2(const double* p, const float* lo, const float* hi) {
+    double s = 0;
+    for (int a = 0; a < 3; ++a) {
+        double d = 0;
+        if (p[a] < lo[a]) d = lo[a] - p[a];
+        else if (p[a] > hi[a]) d = p[a] - hi[a];
+        s += d * d;
+    }
+    return s;
+}
+
+double sceneMinDistance(const gp_scene& s, const double p[3]) {
+    double best = INFINITY;
+    // Depth-first over the pair nodes, nearer child first, pruned by box distance.
+    std::vector<int> stack;
+    stack.push_back(0);
+    auto visitLeaf = [&](int code) {
+        int c = ~code;
+        int first = c >> 3, count = (c & 7) + 1;
+        for (int k = 0; k < count; ++k) {
+            const float* t = &s.leafTris[9 * (size_t)(first + k)];
+            double a[3] = {t[0], t[1], t[2]}, b[3] = {t[3], t[4], t[5]}, cc[3] = {t[6], t[7], t[8]};
+            best = std::min(best, pointTriDist2(p, a, b, cc));
+        }
+    };
+    while (!stack.empty()) {
+        int node = stack.back();
+        stack.pop_back();
+        const float* q = &s.pairs[16 * (size_t)node];
+        int links[2];
+        std::memcpy(&links[0], &q[3], 4);
+        std::memcpy(&links[1], &q[7], 4);
+        double dl = pointBoxDist2(p, q + 0, q + 4), dr = pointBoxDist2(p, q + 8, q + 12);
+        int order[2] = {0, 1};
+        double dd[2] = {dl, dr};
+        if (dr < dl) { order[0] = 1; order[1] = 0; }
+        for (int k = 1; k >= 0; --k) {
+            int side = order[k];
+            if (dd[side] >= best) continue;
+            if (links[side] >= 0) stack.push_back(links[side]);
+            else visitLeaf(links[side]);
+        }
+    }
+    for (const auto& e : s.emitQuads) {
+        const double* o = &e[0];
+        const double* u = &e[3];
+        const double* v = &e[6];
+        double b[3], c[3], d[3];
+        for (int i = 0; i < 3; ++i) { b[i] = o[i] + u[i]; c[i] = o[i] + u[i] + v[i]; d[i] = o[i] + v[i]; }
+        best = std::min(best, pointTriDist2(p, o, b, c));
+        best = std::min(best, pointTriDist2(p, o, c, d));
+    }
+    return std::sqrt(best);
+}
+
+// ------------------------------------------------------------------ film scale
+// The largest value one camera connection can add to one film channel is
+//   B = P * (1/pi) * max_lambda(cie) / (dmin^2 * pixelArea * cosMin^3)
+// with P the photon power (sum of the live per-wavelength weights never exceeds it),
+// dmin the eye's distance to the scene and cosMin the cosine at the film corner.
+// Splats are rounded to the nearest integer unit and saturated at qmax = 2^K, and a
+// photon splats at most (maxDepth + 1) times, so a film sized for N photons holds at
+// most N * (maxDepth + 1) * 2^K <= 2^62 in any channel: no int64 overflow.
+gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
+                      const gp_trace_params& p) {
+    const double splats = (double)p.n_photons * (double)(p.max_depth + 1);
+    if (film->scale > 0.0) {
+        if (film->splatUsed + splats > film->splatBudget * (1.0 + 1e-12))
+            return fail(GP_ERR_CAPACITY, "film photon budget exceeded (create it with a larger max_photons)");
+        film->splatUsed += splats;
+        return GP_OK;
+    }
+    const double dmin = std::max(sceneMinDistance(*s, cb.eye), 1e-6);
+    const double cosMin = 1.0 / std::sqrt(1.0 + cb.tanHalfX * cb.tanHalfX + cb.tanHalfY * cb.tanHalfY);
+    const double cieMax = 1.8;   // max over [360, 830] nm of the three fitted CIE curves is < 1.8
+    const double B = s->basePower * cieMax / 3.141592653589793 /
+                     (dmin * dmin * cb.pixelArea * cosMin * cosMin * cosMin);
+    const double budget = std::max((double)film->maxPhotons, (double)p.n_photons) * (double)(p.max_depth + 1);
+    double K = std::floor(62.0 - std::log2(budget) - 1e-9);
+    K = std::min(K, 52.0);
+    if (K < 8.0) return fail(GP_ERR_CAPACITY, "photon budget too large for a 64-bit film");
+    film->qmax = 1ull << (int)K;
+    film->scale = std::ldexp(1.0, (int)K) / B;
+    film->splatBudget = budget;
+    film->splatUsed = splats;
+    return GP_OK;
+}
+
+gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
+                  const gp_trace_params& p, int slot, DevJob& J) {
+    CamBasis cb;
+    if (!cameraBasis(cam, cb)) return fail(GP_ERR_INVALID, "degenerate camera");
+    if (cam.res_x != film->rx || cam.res_y != film->ry)
+        return fail(GP_ERR_INVALID, "camera resolution differs from the film's");
+    gp_status st = prepareFilm(film, s, cb, p);
+    if (st != GP_OK) return st;
+    Engine& E = engine();
+    std::memset(&J, 0, sizeof J);
+    J.nodes = s->dNodes;
+    J.tris = s->dTris;
+    J.trin = s->dTrin;
+    J.mats = s->dMats;
+    J.ems = s->dEms;
+    J.emCdf = s->dEmCdf;
+    J.nEm = s->nEm;
+    J.basePower = (float)s->basePower;
+    for (int i = 0; i < 3; ++i) {
+        J.eye[i] = (float)cb.eye[i];
+        J.u[i] = (float)cb.u[i];
+        J.v[i] = (float)cb.v[i];
+        J.w[i] = (float)cb.w[i];
+    }
+    J.tanHalfX = (float)cb.tanHalfX;
+    J.tanHalfY = (float)cb.tanHalfY;
+    J.pixelArea = (float)cb.pixelArea;
+    J.resX = cb.resX;
+    J.resY = cb.resY;
+    J.film = (long long*)film->d;
+    J.filmScale = film->scale;
+    J.filmQmax = film->qmax;
+    J.energy = (long long*)(E.dJobEnergy + 5 * (size_t)slot);
+    const double ebits = std::floor(60.0 - std::log2(std::max(1.0, (double)p.n_photons)) - 2.0);
+    J.energyScale = std::ldexp(1.0, (int)std::max(8.0, ebits)) / std::max(s->basePower, 1e-300);
+    J.counters = E.dJobCounters + 4 * (size_t)slot;
+    J.seedBase = p.seed_base;
+    J.salt = p.salt;
+    J.heroC = p.hero_c;
+    J.maxDepth = p.max_depth;
+    J.rrDepth = p.rr_depth;
+    return GP_OK;
+}
+
+// ------------------------------------------------------------------ engine
+gp_tuning defaultTuning() {
+    gp_tuning t;
+    t.queue_capacity = 1 << 20;
+    t.block_extend = 256;
+    t.block_shade = 256;
+    t.grid_waves_per_cu = 32;
+    t.iters_per_graph = 8;
+    t.use_graph = 1;
+    t.lds_stack = 16;
+    t.profile = 0;
+    return t;
+}
+
+Engine& engine() {
+    static Engine e;
+    return e;
+}
+
+void Engine::releaseQueues() {
+    if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+    for (int i = 0; i < nbufs; ++i) hipFree(bufs[i]);
+    nbufs = 0;
+    if (dCtrl) hipFree(dCtrl);
+    if (dJobs) hipFree(dJobs);
+    if (hCtrl) hipHostFree(hCtrl);
+    if (hJobs) hipHostFree(hJobs);
+    if (dJobEnergy) hipFree(dJobEnergy);
+    if (dJobCounters) hipFree(dJobCounters);
+    dCtrl = nullptr; dJobs = nullptr; hCtrl = nullptr; hJobs = nullptr;
+    dJobEnergy = nullptr; dJobCounters = nullptr;
+    int2* sp = q.spill;
+    q = Queues{};
+    q.spill = sp;
+    q.spillDepth = spillDepth;
+    capacity = 0;
+}
+
+void Engine::releaseSpill() {
+    if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+    if (q.spill) hipFree(q.spill);
+    q.spill = nullptr;
+    q.spillDepth = 0;
+    spillDepth = 0;
+    spillThreads = 0;
+}
+
+void Engine::release() {
+    releaseQueues();
+    releaseSpill();
+}
+
+gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
+    if (!tunSet) { tun = defaultTuning(); tunSet = true; }
+    DeviceState& d = device();
+    const int cus = d.cus > 0 ? d.cus : 256;
+    ls.threadsExtend = 256;
+    ls.threadsShade = 256;
+    ls.blocksExtend = std::max(1, cus * tun.grid_waves_per_cu / 4);
+    ls.blocksShade = std::max(1, cus * tun.grid_waves_per_cu / 4);
+    const size_t threads = (size_t)ls.blocksExtend * 256;
+    const bool needQueues = (cap != capacity) || !dCtrl;
+    const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
+    if (!needQueues && !needSpillAlloc && q.lds == lds && graphShape == ls.blocksExtend) return GP_OK;
+    if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+    if (needQueues) {
+        releaseQueues();
+        auto alloc = [&](size_t bytes, void** out) -> gp_status {
+            hipError_t e = hipMalloc(out, bytes);
+            if (e != hipSuccess) return fail(GP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            bufs[nbufs++] = *out;
+            return GP_OK;
+        };
+        gp_status st;
+        for (int k = 0; k < 2; ++k) {
+            if ((st = alloc(cap * sizeof(float4), (void**)&q.ro[k])) != GP_OK) return st;
+            if ((st = alloc(cap * sizeof(float4), (void**)&q.rd[k])) != GP_OK) return st;
+            if ((st = alloc(cap * sizeof(float4), (void**)&q.lam[k])) != GP_OK) return st;
+            if ((st = alloc(cap * sizeof(float4), (void**)&q.beta[k])) != GP_OK) return st;
+            if ((st = alloc(cap * sizeof(uint4), (void**)&q.st[k])) != GP_OK) return st;
+        }
+        if ((st = alloc(cap * sizeof(float2), (void**)&q.hit)) != GP_OK) return st;
+        // Connections per iteration: one per emitted photon (direct view of the light)
+        // plus at most one per live path, and live paths include this iteration's
+        // emissions, so the connection queue holds twice the path capacity.
+        const size_t ccap = 2 * (size_t)cap;
+        if ((st = alloc(ccap * sizeof(float4), (void**)&q.co)) != GP_OK) return st;
+        if ((st = alloc(ccap * sizeof(float4), (void**)&q.cd)) != GP_OK) return st;
+        if ((st = alloc(ccap * sizeof(float4), (void**)&q.cx)) != GP_OK) return st;
+        GP_HIP(hipMalloc(&dCtrl, sizeof(Ctrl)));
+        GP_HIP(hipMalloc(&dJobs, sizeof(DevJob) * kMaxJobs));
+        GP_HIP(hipHostMalloc(&hCtrl, sizeof(Ctrl)));
+        GP_HIP(hipHostMalloc(&hJobs, sizeof(DevJob) * kMaxJobs));
+        GP_HIP(hipMalloc(&dJobEnergy, sizeof(unsigned long long) * 5 * kMaxJobs));
+        GP_HIP(hipMalloc(&dJobCounters, sizeof(unsigned long long) * 4 * kMaxJobs));
+        std::memset(hCtrl, 0, sizeof(Ctrl));
+        std::memset(hJobs, 0, sizeof(DevJob) * kMaxJobs);
+        capacity = cap;
+        q.ctrl = dCtrl;
+        q.jobs = dJobs;
+    }
+    if (needSpillAlloc) {
+        const int depthAlloc = std::max(needSpill, spillDepth);
+        if (q.spill) hipFree(q.spill);
+        q.spill = nullptr;
+        hipError_t e = hipMalloc(&q.spill, threads * (size_t)depthAlloc * sizeof(int2));
+        if (e != hipSuccess) { spillDepth = 0; spillThreads = 0; return fail(GP_ERR_OOM, "spill stack allocation failed"); }
+        spillDepth = depthAlloc;
+        spillThreads = threads;
+    }
+    q.spillDepth = spillDepth;
+    q.lds = lds;
+    graphShape = ls.blocksExtend;
+    return GP_OK;
+}
+
+gp_status Engine::enqueueIterations(int n) {
+    hipStream_t s = device().stream;
+    const bool prof = tun.profile != 0;
+    auto timed = [&](int k, auto&& launch) -> gp_status {
+        if (!prof) { launch(); return GP_OK; }
+        std::array<hipEvent_t, 2> ev;
+        GP_HIP(hipEventCreate(&ev[0]));
+        GP_HIP(hipEventCreate(&ev[1]));
+        GP_HIP(hipEventRecord(ev[0], s));
+        launch();
+        GP_HIP(hipEventRecord(ev[1], s));
+        pending[k].push_back(ev);
+        return GP_OK;
+    };
+    gp_status st;
+    for (int i = 0; i < n; ++i) {
+        const int cur = parity;
+        if ((st = timed(0, [&] { launchSchedule(q, cur, s); })) != GP_OK) return st;
+        if ((st = timed(1, [&] { launchEmit(q, cur, ls, s); })) != GP_OK) return st;
+        if ((st = timed(2, [&] { launchExtend(q, cur, ls, s); })) != GP_OK) return st;
+        if ((st = timed(3, [&] { launchShade(q, cur, ls, s); })) != GP_OK) return st;
+        if ((st = timed(4, [&] { launchConnect(q, ls, s); })) != GP_OK) return st;
+        parity ^= 1;
+    }
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+void Engine::resetProfile() {
+    for (int k = 0; k < 5; ++k) {
+        for (auto& ev : pending[k]) { hipEventDestroy(ev[0]); hipEventDestroy(ev[1]); }
+        pending[k].clear();
+        kernelMs[k] = 0.0;
+        kernelLaunches[k] = 0;
+    }
+}
+
+gp_status Engine::collectProfile() {
+    for (int k = 0; k < 5; ++k) {
+        for (auto& ev : pending[k]) {
+            float ms = 0.f;
+            GP_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            kernelMs[k] += ms;
+            kernelLaunches[k] += 1;
+            hipEventDestroy(ev[0]);
+            hipEventDestroy(ev[1]);
+        }
+        pending[k].clear();
+    }
+    return GP_OK;
+}
+
+gp_status Engine::runIterations(int n) {
+    hipStream_t s = device().stream;
+    const int K = std::max(2, tun.iters_per_graph & ~1);
+    if (!tun.use_graph || tun.profile || parity != 0 || n < K) return enqueueIterations(n);
+    if (!graph || graphIters != K || graphLds != q.lds) {
+        if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+        hipGraph_t g;
+        GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int saved = parity;
+        gp_status st = enqueueIterations(K);
+        hipError_t ce = hipStreamEndCapture(s, &g);
+        parity = saved;
+        if (st != GP_OK) return st;
+        if (ce != hipSuccess) return fail(GP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ie != hipSuccess) { graph = nullptr; return fail(GP_ERR_HIP, "hipGraphInstantiate failed"); }
+        graphIters = K;
+        graphLds = q.lds;
+    }
+    int done = 0;
+    while (n - done >= K) {
+        GP_HIP(hipGraphLaunch(graph, s));
+        done += K;
+    }
+    if (done < n) return enqueueIterations(n - done);
+    return GP_OK;
+}
+
+gp_status Engine::readCtrl() {
+    hipStream_t s = device().stream;
+    GP_HIP(hipMemcpyAsync(hCtrl, dCtrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (tun.profile) return collectProfile();
+    return GP_OK;
+}
+
+}  // namespace gp

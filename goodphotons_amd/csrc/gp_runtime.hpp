@@ -1,0 +1,126 @@
+// Host runtime internals shared by the C ABI implementation files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/goodphotons.h"
+#include "gp_kernels.hpp"
+#include "gp_types.hpp"
+
+struct gp_scene {
+    int64_t nTris = 0;
+    int64_t nNodes = 0;
+    int depth = 0;
+    float4* dNodes = nullptr;
+    float4* dTris = nullptr;
+    float4* dTrin = nullptr;
+    gp::DevMaterial* dMats = nullptr;
+    gp::DevEmitter* dEms = nullptr;
+    double* dCdfs = nullptr;
+    double* dEmCdf = nullptr;
+    int nEm = 0;
+    double totalPower = 0.0;
+    double basePower = 0.0;
+    int64_t deviceBytes = 0;
+    double buildMs = 0.0;
+    // host copies for camera-distance bounds
+    std::vector<float> pairs;          // 16 floats per pair node
+    std::vector<float> leafTris;       // 9 floats per triangle, leaf order
+    std::vector<std::array<double, 12>> emitQuads;   // origin, u, v, normal
+};
+
+struct gp_film {
+    int32_t rx = 0, ry = 0;
+    int64_t maxPhotons = 0;
+    unsigned long long* d = nullptr;
+    double scale = 0.0;                 // contribution -> integer units (0 = not yet fixed)
+    unsigned long long qmax = 0;        // largest single splat value
+    double splatBudget = 0.0;           // photons*(maxDepth+1) the scale was sized for
+    double splatUsed = 0.0;
+};
+
+namespace gp {
+
+gp_status fail(gp_status s, const std::string& msg);
+#define GP_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::gp::fail(GP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DeviceState {
+    bool ok = false;
+    gp_status status = GP_ERR_NO_DEVICE;
+    std::string why;
+    int id = 0;
+    int cus = 0;
+    size_t ldsPerBlock = 0;
+    std::string name, arch;
+    hipStream_t stream = nullptr;
+};
+DeviceState& device();
+gp_status requireDevice();
+
+// Camera basis as the reference's Camera::lookAt (src/camera.h:110-129), in double.
+struct CamBasis {
+    double eye[3], u[3], v[3], w[3];
+    double tanHalfX, tanHalfY, pixelArea;
+    int resX, resY;
+};
+bool cameraBasis(const gp_camera_desc& c, CamBasis& out);
+
+// Smallest distance from point p to any triangle or emitter quad of the scene.
+double sceneMinDistance(const gp_scene& s, const double p[3]);
+
+// Wavefront engine: device queues, control block, job table, graphs.
+struct Engine {
+    std::mutex mu;
+    gp_tuning tun{};
+    bool tunSet = false;
+    unsigned capacity = 0;
+    int spillDepth = 0;
+    size_t spillThreads = 0;
+    Queues q{};
+    LaunchShape ls{};
+    Ctrl* dCtrl = nullptr;
+    Ctrl* hCtrl = nullptr;              // pinned
+    DevJob* dJobs = nullptr;
+    DevJob* hJobs = nullptr;            // pinned
+    unsigned long long* dJobEnergy = nullptr;    // kMaxJobs * 5
+    unsigned long long* dJobCounters = nullptr;  // kMaxJobs * 4
+    void* bufs[16] = {};
+    int nbufs = 0;
+    int parity = 0;
+    hipGraphExec_t graph = nullptr;
+    int graphIters = 0;
+    int graphLds = 0;
+    int graphShape = 0;
+    // per-kernel HIP-event timing (tuning.profile)
+    std::vector<std::array<hipEvent_t, 2>> pending[5];
+    double kernelMs[5] = {};
+    uint64_t kernelLaunches[5] = {};
+    void resetProfile();
+    gp_status collectProfile();
+
+    gp_status ensure(unsigned cap, int lds, int spillDepth);
+    void release();
+    void releaseQueues();
+    void releaseSpill();
+    gp_status enqueueIterations(int n);       // plain launches, tracks parity
+    gp_status runIterations(int n);           // graph replay when possible
+    gp_status readCtrl();                     // synchronous copy of the control block
+};
+Engine& engine();
+gp_tuning defaultTuning();
+
+gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
+                  const gp_trace_params& p, int slot, DevJob& J);
+gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
+                      const gp_trace_params& p);
+
+}  // namespace gp

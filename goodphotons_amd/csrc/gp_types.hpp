@@ -1,0 +1,108 @@
+// Plain-old-data layouts shared by the host runtime and the CDNA4 kernels.
+// Everything the kernels read lives in HBM in these forms:
+//   BVH        child-pair nodes, 64 B each (both children's boxes + links)
+//   triangles  3 x float4 per triangle in leaf order (vertices; .w = prim id,
+//              material id), plus 1 x float4 geometric normal (.w = material)
+//   paths      structure-of-arrays queues of float4/uint4 (80 B per path)
+//   hits       one float2 per path (t, leaf-order triangle index)
+//   camera     connections: 3 x float4 per record (48 B)
+//   films      int64 fixed-point (X, Y, Z, hits) per pixel, 32 B
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gp {
+
+constexpr int kMaxHero = 4;           // wavelengths per path (hero + 3)
+constexpr int kMaxJobs = 256;         // job slots in one wavefront
+constexpr int kLambdaBins = 470;      // 1-nm bins over [360, 830)
+constexpr float kRayEps = 1e-6f;      // origin offset and tmin, as the reference CPU path
+constexpr int kLeafMax = 4;           // triangles per BVH leaf
+
+// Leaf links are negative: ~((first << 3) | (count - 1)).
+inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (count - 1)); }
+
+struct DevSpectrum {
+    int32_t kind;
+    float p[6];
+    float band_lo, band_hi;
+    int32_t pad[3];
+};  // 48 B
+
+struct DevMaterial {
+    int32_t type;
+    float roughness;
+    int32_t pad[2];
+    DevSpectrum reflect;
+    DevSpectrum ior;
+};  // 112 B
+
+struct DevEmitter {
+    float origin[3], u[3], v[3], normal[3], beam[3];
+    int32_t collimated;
+    float power;        // integral(SPD) * area * pi
+    int32_t pad;
+    const double* cdf;  // kLambdaBins + 1 entries, normalised
+};
+
+// One job = one (scene, camera, film, photon budget). A single gp_trace is one job;
+// the batched and serving paths run many jobs in the same queues.
+struct DevJob {
+    // scene
+    const float4* nodes;
+    const float4* tris;
+    const float4* trin;
+    const DevMaterial* mats;
+    const DevEmitter* ems;
+    const double* emCdf;     // power CDF over emitters (n_em entries)
+    int32_t nEm;
+    float basePower;         // photon power: the single emitter's, else the total
+    // camera (Camera::lookAt basis, float)
+    float eye[3], u[3], v[3], w[3];
+    float tanHalfX, tanHalfY, pixelArea;
+    int32_t resX, resY;
+    // film + ledger
+    long long* film;         // res*4 int64
+    double filmScale;        // contribution -> integer units
+    unsigned long long filmQmax;   // saturation value of one splat
+    long long* energy;       // 5 int64 (emitted, absorbed, sensor, escaped, residual)
+    double energyScale;
+    unsigned long long* counters;  // photons, segments, connections, splats
+    // path settings
+    unsigned long long seedBase, salt;
+    int32_t heroC, maxDepth, rrDepth, pad0;
+};
+
+// Device-side wavefront control block.
+struct Ctrl {
+    unsigned int nPaths[2];     // paths in queue 0 / 1
+    unsigned int nConn;         // camera connections queued this iteration
+    unsigned int nEmit;         // photons emitted this iteration
+    unsigned int emitBase;      // queue slot of the first emitted photon
+    unsigned int iter;          // iterations run
+    int32_t nJobs;              // unused slots above this are never read
+    unsigned int capacity;      // path queue capacity
+    unsigned int connCapacity;
+    unsigned int pad[3];
+    // per job
+    unsigned long long nextPhoton[kMaxJobs];   // next photon index to emit (job-relative)
+    unsigned long long endPhoton[kMaxJobs];    // one past the last photon
+    int32_t lastEmitIter[kMaxJobs];            // iteration that emitted the job's last photon
+    int32_t active[kMaxJobs];                  // 1 = emitting / in flight
+    int32_t order[kMaxJobs];                   // job slots in emission priority order
+    int32_t nOrder;                            // entries of `order` in use
+    int32_t pad2[3];
+    // this iteration's emission, indexed by position k in `order`
+    unsigned int emitPrefix[kMaxJobs + 1];     // prefix of photons emitted before order[k]
+    int32_t emitJob[kMaxJobs];                 // = order[k]
+    unsigned long long emitStart[kMaxJobs];    // first photon index emitted for order[k]
+};
+
+// Path metadata word (uint4.w of the state array).
+//   bits  0..15 job slot, 16..23 bounce, 24..26 live wavelengths, 31 dead
+inline constexpr unsigned packMeta(unsigned job, unsigned bounce, unsigned nUp) {
+    return job | (bounce << 16) | (nUp << 24);
+}
+
+}  // namespace gp

@@ -1,0 +1,294 @@
+/*
+ * goodphotons.h - C ABI of the MI355X-native forward light-tracing core.
+ *
+ * Plain C: fixed-width integers, doubles, pointers and sizes. No HIP, CUDA or
+ * torch types appear here, so the renderer's host (C++20, or Python through
+ * ctypes) can bind it without a device compiler.
+ *
+ * What each entry point replaces in the reference (inhahe/GoodPhotons, ftrace
+ * v0.190.4, paths relative to its repository root):
+ *
+ *   gp_device_query        <- cudaAvailable / cudaDeviceName / cudaMemInfo
+ *                             (src/render_cuda.h:24, :27, :32)
+ *   gp_scene_create        <- buildUpload / buildUploadScene + cudaForwardSupported
+ *                             (src/render_cuda.cu:13600, :12692-13548, :12504-12631)
+ *   gp_scene_destroy       <- freeUpload (src/render_cuda.cu:12683)
+ *   gp_film_*              <- the device film inside renderForwardCuda
+ *                             (src/render_cuda.cu:13722-13746) and the resident
+ *                             films of SharedGpuSession (src/render_cuda.h:121-143)
+ *   gp_trace               <- renderForwardCuda (src/render_cuda.h:79-82,
+ *                             src/render_cuda.cu:13712-13756) and
+ *                             sharedForwardGpuBatch (src/render_cuda.h:136-137)
+ *   gp_probe_closest_hit   <- Scene::closestHit / Bvh::traverseClosest
+ *                             (src/scene.h:2194, src/bvh.h:180-224) exposed as a
+ *                             batch probe in the style of -checklattice
+ *                             (src/render_cuda.h:446-464)
+ *   gp_probe_any_hit       <- Scene::occluded / Bvh::traverseAny
+ *                             (src/scene.h:2249, src/bvh.h:228-249)
+ *   gp_server_*            <- no single counterpart: the resident session
+ *                             (sharedForwardGpuBegin/Batch/Download/End,
+ *                             src/render_cuda.h:121-143) generalised to many
+ *                             scenes, films and jobs in flight at once.
+ *
+ * Errors: every entry point returns gp_status. GP_OK is 0. On failure
+ * gp_last_error() returns a message for the calling thread. Nothing here
+ * calls exit(); the reference aborts the process instead
+ * (src/render_cuda.cu:99-121).
+ */
+#ifndef GOODPHOTONS_H
+#define GOODPHOTONS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_ABI_VERSION 1
+
+typedef enum gp_status {
+    GP_OK = 0,
+    GP_ERR_INVALID = 1,      /* bad argument (null pointer, size out of range) */
+    GP_ERR_UNSUPPORTED = 2,  /* scene feature outside the supported set */
+    GP_ERR_NO_DEVICE = 3,    /* no usable gfx950 device */
+    GP_ERR_HIP = 4,          /* a HIP runtime call failed */
+    GP_ERR_OOM = 5,          /* device memory exhausted */
+    GP_ERR_CAPACITY = 6,     /* a fixed limit (stack depth, jobs, scratch) exceeded */
+    GP_ERR_REJECTED = 7,     /* admission control refused a job */
+    GP_ERR_STATE = 8         /* call not valid in the object's current state */
+} gp_status;
+
+/* ---------------------------------------------------------------------------
+ * Spectra. Analytic forms only (no sampled tables), so a step edge such as the
+ * red wall's 600 nm switch is exact on both host and device.
+ * Wavelengths are in nm. A band [band_lo, band_hi] multiplies the curve by an
+ * indicator when band_hi > band_lo (used to restrict an emitter's SPD).
+ * ------------------------------------------------------------------------- */
+enum {
+    GP_SPEC_CONST = 0,      /* v = p[0]                                            */
+    GP_SPEC_STEP = 1,       /* v = (lam < p[2]) ? p[0] : p[1]                      */
+    GP_SPEC_GAUSS = 2,      /* v = p[0] + p[1]*exp(-0.5*((lam-p[2])/p[3])^2)       */
+    GP_SPEC_BLACKBODY = 3,  /* v = p[1] * Planck(T = p[0] K, lam)  (SI, per metre) */
+    GP_SPEC_SELLMEIER = 4,  /* n = sqrt(max(1, 1 + sum Bi l^2/(l^2-Ci))), l in um,
+                               B = p[0..2], C = p[3..5] (um^2)                      */
+    GP_SPEC_CAUCHY = 5      /* n = p[0] + p[1]/l^2, l in um                         */
+};
+
+typedef struct gp_spectrum {
+    int32_t kind;
+    int32_t reserved;
+    double p[6];
+    double band_lo, band_hi;
+} gp_spectrum;
+
+/* Material types use the reference's MatType numbering (src/scene.h:42). */
+enum {
+    GP_MAT_DIFFUSE = 0,
+    GP_MAT_DIELECTRIC = 1,
+    GP_MAT_MIRROR = 2,
+    GP_MAT_GLOSSY = 4
+};
+
+typedef struct gp_material {
+    int32_t type;
+    int32_t reserved;
+    double roughness;        /* Glossy lobe width; on a Dielectric it frosts both lobes */
+    gp_spectrum reflect;     /* diffuse albedo, or mirror/glossy tint                    */
+    gp_spectrum ior;         /* Dielectric index of refraction                           */
+} gp_material;
+
+/* Quad area emitter (Scene::addAreaLight, src/scene.h:1306). The point is
+ * origin + u*s + v*t for s,t in [0,1); emission is cosine-weighted about
+ * `normal` unless `collimated`, in which case every photon leaves along
+ * beam_dir. Power = integral(SPD) * area * pi over 1-nm bins in [360, 830). */
+typedef struct gp_emitter {
+    double origin[3], u[3], v[3], normal[3];
+    double area;
+    int32_t collimated;
+    int32_t reserved;
+    double beam_dir[3];
+    gp_spectrum spd;
+} gp_emitter;
+
+/* Triangle-only scene. Global primitive id = index into the triangle list,
+ * exactly as the reference numbers prims (tris first, src/scene.h:2005-2044). */
+typedef struct gp_scene_desc {
+    int64_t n_tris;
+    int32_t n_materials;
+    int32_t n_emitters;
+    const float* verts;          /* n_tris * 9 floats: v0.xyz v1.xyz v2.xyz */
+    const int32_t* mat_ids;      /* n_tris                                    */
+    const gp_material* materials;
+    const gp_emitter* emitters;
+} gp_scene_desc;
+
+/* Pinhole camera as Camera::lookAt builds it (src/camera.h:110-129). */
+typedef struct gp_camera_desc {
+    double eye[3], target[3], up[3];
+    double fov_y_deg;
+    int32_t res_x, res_y;
+} gp_camera_desc;
+
+/* Energy ledger, same fields as EnergyReport (src/render.h:29-31). */
+typedef struct gp_energy {
+    double emitted, absorbed, sensor, escaped, residual;
+} gp_energy;
+
+typedef struct gp_trace_params {
+    int64_t n_photons;       /* photons to trace in this call                        */
+    uint64_t seed_base;      /* absolute index of the first photon (resume offset)   */
+    uint64_t salt;           /* seeding salt; 0x9E3779B97F4A7C15 is the reference's  */
+    int32_t hero_c;          /* wavelengths per path, 1..4                           */
+    int32_t max_depth;       /* closest-hit segments per path (Renderer::maxBounce)  */
+    int32_t rr_depth;        /* first bounce with Russian roulette; 0 = every vertex */
+    int32_t flags;           /* reserved, 0                                          */
+} gp_trace_params;
+
+typedef struct gp_trace_stats {
+    uint64_t photons;        /* photons emitted                                      */
+    uint64_t segments;       /* closest-hit queries (one per bounce-loop iteration)  */
+    uint64_t connections;    /* camera shadow rays cast (any-hit queries)            */
+    uint64_t splats;         /* connections that reached the film                    */
+    uint64_t iterations;     /* wavefront iterations run                             */
+    double wall_ms;          /* host wall time of the trace loop                     */
+    /* With gp_tuning.profile = 1: summed device time and launch count per kernel,
+     * index 0 schedule, 1 emit, 2 extend (closest hit), 3 shade, 4 connect. */
+    double kernel_ms[5];
+    uint64_t kernel_launches[5];
+} gp_trace_stats;
+
+typedef struct gp_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t wave_size;
+    int64_t total_mem;
+    int64_t free_mem;
+    int32_t lds_per_block;
+    int32_t reserved;
+} gp_device_info;
+
+typedef struct gp_scene_info {
+    int64_t n_tris;
+    int64_t n_nodes;
+    int32_t bvh_depth;       /* longest root-to-leaf path                            */
+    int32_t n_emitters;
+    int64_t device_bytes;    /* BVH + triangles + tables resident in HBM             */
+    double build_ms;         /* host BVH build time                                  */
+    double total_power;
+} gp_scene_info;
+
+typedef struct gp_scene gp_scene;
+typedef struct gp_film gp_film;
+typedef struct gp_server gp_server;
+
+/* ---- library ------------------------------------------------------------ */
+int32_t gp_abi_version(void);
+const char* gp_status_string(gp_status s);
+const char* gp_last_error(void);
+gp_status gp_device_query(gp_device_info* out);
+
+/* ---- scenes ------------------------------------------------------------- */
+/* Validates the descriptor, builds the BVH on the host and uploads BVH,
+ * triangles, materials and emitter tables. Fails with GP_ERR_UNSUPPORTED for
+ * features outside the set above (the reference's cudaForwardSupported gate). */
+gp_status gp_scene_create(const gp_scene_desc* desc, gp_scene** out);
+gp_status gp_scene_get_info(const gp_scene* scene, gp_scene_info* out);
+void gp_scene_destroy(gp_scene* scene);
+
+/* ---- films -------------------------------------------------------------- */
+/* A film is res_x*res_y pixels of int64 fixed-point (X, Y, Z, hits). The
+ * fixed-point scale is fixed by the first gp_trace into the film; it is
+ * chosen so that max_photons photons cannot overflow any pixel (pass 0 for
+ * the default of 2^32). Integer accumulation makes the film bit-identical
+ * run to run and independent of scheduling. */
+gp_status gp_film_create(int32_t res_x, int32_t res_y, int64_t max_photons, gp_film** out);
+gp_status gp_film_reset(gp_film* film);
+/* xyz: res_x*res_y*3 doubles (un-normalised sums, pixel-major, row py*res_x+px);
+ * hits: res_x*res_y doubles. Either may be null. */
+gp_status gp_film_download(gp_film* film, double* xyz, double* hits);
+/* Raw fixed-point words, res_x*res_y*4 int64 (X, Y, Z, hits); scale_out gets the
+ * factor that maps a double contribution to one integer unit. */
+gp_status gp_film_download_raw(gp_film* film, int64_t* words, double* scale_out);
+void gp_film_destroy(gp_film* film);
+
+/* ---- tracing ------------------------------------------------------------ */
+/* Trace params->n_photons photons of `scene` through `cam` into `film`
+ * (blocking). Energy is ACCUMULATED into *energy (+=), as renderForwardCuda
+ * accumulates into eOut. stats may be null. */
+gp_status gp_trace(gp_scene* scene, const gp_camera_desc* cam, gp_film* film,
+                   const gp_trace_params* params, gp_energy* energy,
+                   gp_trace_stats* stats);
+
+/* Closest hit for n rays (host arrays). rays: n*6 floats (origin.xyz, dir.xyz).
+ * prim_out: global primitive id or -1; t_out: hit distance (inf on miss). */
+gp_status gp_probe_closest_hit(gp_scene* scene, const float* rays, int64_t n,
+                               float tmin, int32_t* prim_out, float* t_out);
+/* Any hit on [tmin, tmax_i) for n rays: rays n*6 floats, tmax n floats,
+ * occluded_out n bytes (0/1). */
+gp_status gp_probe_any_hit(gp_scene* scene, const float* rays, const float* tmax,
+                           int64_t n, float tmin, uint8_t* occluded_out);
+
+/* ---- wavefront tuning --------------------------------------------------- */
+typedef struct gp_tuning {
+    int32_t queue_capacity;  /* paths in flight (power of two)                       */
+    int32_t block_extend;    /* threads per block of the traversal kernels           */
+    int32_t block_shade;     /* threads per block of emit/shade                      */
+    int32_t grid_waves_per_cu; /* resident waves per CU targeted by the fixed grids  */
+    int32_t iters_per_graph; /* wavefront iterations per captured hipGraph           */
+    int32_t use_graph;       /* 1 = replay captured graphs, 0 = plain launches       */
+    int32_t lds_stack;       /* traversal stack entries per lane held in LDS (8..32)  */
+    int32_t profile;         /* 1 = time every kernel launch with HIP events (no graphs) */
+} gp_tuning;
+
+gp_status gp_get_tuning(gp_tuning* out);
+gp_status gp_set_tuning(const gp_tuning* in);
+/* Times candidate settings on `scene`/`cam` with a short trace and installs
+ * the fastest. */
+gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam,
+                      const gp_trace_params* params, gp_tuning* best_out);
+
+/* ---- serving: many jobs in one wavefront -------------------------------- */
+typedef struct gp_server_config {
+    int64_t hbm_budget_bytes; /* admission limit for films + queues of live jobs   */
+    int32_t max_live_jobs;    /* job slots in the device job table (<= 256)        */
+    int32_t reserved;
+} gp_server_config;
+
+typedef struct gp_job_desc {
+    const gp_scene* scene;
+    gp_camera_desc camera;
+    gp_trace_params params;
+    int64_t user_tag;
+} gp_job_desc;
+
+typedef struct gp_job_result {
+    int64_t job_id;
+    int64_t user_tag;
+    int32_t status;          /* gp_status of the job                                 */
+    int32_t res_x, res_y;
+    gp_energy energy;
+    gp_trace_stats stats;
+    double submit_ms, start_ms, finish_ms;   /* server clock, ms                     */
+} gp_job_result;
+
+gp_status gp_server_create(const gp_server_config* cfg, gp_server** out);
+/* Queue a job. It is admitted into the running wavefront as soon as a job slot
+ * and its HBM footprint are free. Returns its id in *job_id. */
+gp_status gp_server_submit(gp_server* srv, const gp_job_desc* job, int64_t* job_id);
+/* Advance the wavefront by at most max_iterations iterations (0 = until every
+ * queued job has finished). Returns the number of jobs finished so far. */
+gp_status gp_server_step(gp_server* srv, int32_t max_iterations, int32_t* finished_out);
+/* Fetch a finished job's result and film (xyz res*3, hits res, either may be
+ * null). Returns GP_ERR_STATE if the job is not finished. The job's device
+ * film is released after this call. */
+gp_status gp_server_take_result(gp_server* srv, int64_t job_id, gp_job_result* res,
+                                double* xyz, double* hits);
+gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live);
+void gp_server_destroy(gp_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GOODPHOTONS_H */

@@ -1,0 +1,273 @@
+"""Test infrastructure only: ctypes wrappers for the CPU checkers.
+
+  Oracle()      this repository's double-precision restatement (liboracle.so)
+  Reference()   the reference renderer's own forward path compiled from its
+                headers (oracle/_ref/libref.so; built only where the reference
+                sources exist, travels as a built .so)
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
+package, and only as the checker / baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from goodphotons_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+def build(with_ref: bool = True) -> None:
+    target = "all" if with_ref else os.path.join(HERE, "liboracle.so")
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile"), target], check=True)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class _CpuRenderer:
+    prefix = ""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run oracle build)")
+        self.lib = C.CDLL(path)
+        p = self.prefix
+        L = self.lib
+        f = getattr(L, p + "scene_create"); f.restype = C.c_void_p; f.argtypes = [C.POINTER(abi.SceneDesc)]
+        f = getattr(L, p + "scene_destroy"); f.restype = None; f.argtypes = [C.c_void_p]
+        f = getattr(L, p + "scene_bvh_nodes"); f.restype = C.c_int64; f.argtypes = [C.c_void_p]
+        f = getattr(L, p + "scene_total_power"); f.restype = C.c_double; f.argtypes = [C.c_void_p]
+        f = getattr(L, p + "closest_hit")
+        f.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, _i32p, _dp, _i64p, _i64p]
+        f = getattr(L, p + "closest_hit_linear")
+        f.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, _i32p, _dp]
+        f = getattr(L, p + "occluded"); f.argtypes = [C.c_void_p, _dp, _dp, C.c_int64, _u8p]
+        f = getattr(L, p + "pcg_stream"); f.restype = None
+        f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _u32p]
+        f = getattr(L, p + "seed_unit_uniforms"); f.restype = None
+        f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _dp]
+        f = getattr(L, p + "mix64"); f.restype = C.c_uint64; f.argtypes = [C.c_uint64]
+        f = getattr(L, p + "spectrum_eval"); f.restype = None
+        f.argtypes = [C.POINTER(abi.Spectrum), _dp, C.c_int, _dp]
+        f = getattr(L, p + "cie"); f.restype = None; f.argtypes = [_dp, C.c_int, _dp]
+        f = getattr(L, p + "cie_y_integral"); f.restype = C.c_double; f.argtypes = []
+        f = getattr(L, p + "emission_sample"); f.restype = None
+        f.argtypes = [C.POINTER(abi.Spectrum), _dp, C.c_int, _dp, _dp, _dp]
+        f = getattr(L, p + "sample_bundle"); f.restype = C.c_int
+        f.argtypes = [C.POINTER(abi.Spectrum), C.c_double, C.c_int, _dp, _dp]
+        f = getattr(L, p + "cosine_hemisphere"); f.restype = None
+        f.argtypes = [_dp, C.c_uint64, C.c_uint64, C.c_int, _dp]
+        f = getattr(L, p + "sample_glossy"); f.restype = None
+        f.argtypes = [_dp, C.c_double, C.c_uint64, C.c_uint64, C.c_int, _dp]
+        f = getattr(L, p + "camera_basis"); f.restype = None
+        f.argtypes = [C.POINTER(abi.CameraDesc), _dp]
+        f = getattr(L, p + "camera_project"); f.restype = C.c_int
+        f.argtypes = [C.POINTER(abi.CameraDesc), _dp, C.c_int, _i32p, _i32p, _dp, _dp, _dp]
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    # ---- scenes
+    def scene(self, sd) -> "CpuScene":
+        return CpuScene(self, sd)
+
+    # ---- primitives
+    def pcg_stream(self, seq, seed, n):
+        out = np.zeros(n, np.uint32)
+        self._f("pcg_stream")(seq, seed, n, _ptr(out, _u32p))
+        return out
+
+    def seed_unit_uniforms(self, k, salt, n):
+        out = np.zeros(n, np.float64)
+        self._f("seed_unit_uniforms")(k, salt, n, _ptr(out, _dp))
+        return out
+
+    def mix64(self, x):
+        return int(self._f("mix64")(x))
+
+    def spectrum_eval(self, spec, lam):
+        lam = np.ascontiguousarray(lam, np.float64)
+        out = np.zeros_like(lam)
+        self._f("spectrum_eval")(C.byref(spec), _ptr(lam, _dp), len(lam), _ptr(out, _dp))
+        return out
+
+    def cie(self, lam):
+        lam = np.ascontiguousarray(lam, np.float64)
+        out = np.zeros((len(lam), 3))
+        self._f("cie")(_ptr(lam, _dp), len(lam), _ptr(out, _dp))
+        return out
+
+    def cie_y_integral(self):
+        return float(self._f("cie_y_integral")())
+
+    def emission_sample(self, spec, u):
+        u = np.ascontiguousarray(u, np.float64)
+        lam = np.zeros_like(u)
+        pdf = np.zeros_like(u)
+        integral = C.c_double(0)
+        self._f("emission_sample")(C.byref(spec), _ptr(u, _dp), len(u), _ptr(lam, _dp),
+                                   _ptr(pdf, _dp), C.byref(integral))
+        return lam, pdf, integral.value
+
+    def sample_bundle(self, spec, u, c):
+        lam = np.zeros(8)
+        pdf = np.zeros(8)
+        ok = self._f("sample_bundle")(C.byref(spec), u, c, _ptr(lam, _dp), _ptr(pdf, _dp))
+        return bool(ok), lam[:c], pdf[:c]
+
+    def cosine_hemisphere(self, nrm, seq, seed, n):
+        nrm = np.ascontiguousarray(nrm, np.float64)
+        out = np.zeros((n, 3))
+        self._f("cosine_hemisphere")(_ptr(nrm, _dp), seq, seed, n, _ptr(out, _dp))
+        return out
+
+    def sample_glossy(self, mdir, rough, seq, seed, n):
+        mdir = np.ascontiguousarray(mdir, np.float64)
+        out = np.zeros((n, 3))
+        self._f("sample_glossy")(_ptr(mdir, _dp), rough, seq, seed, n, _ptr(out, _dp))
+        return out
+
+    def camera_basis(self, cam):
+        out = np.zeros(14)
+        self._f("camera_basis")(C.byref(cam), _ptr(out, _dp))
+        return out
+
+    def camera_project(self, cam, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = len(pts)
+        px = np.zeros(n, np.int32); py = np.zeros(n, np.int32)
+        cc = np.zeros(n); d2 = np.zeros(n); om = np.zeros(n)
+        self._f("camera_project")(C.byref(cam), _ptr(pts, _dp), n, _ptr(px, _i32p), _ptr(py, _i32p),
+                                  _ptr(cc, _dp), _ptr(d2, _dp), _ptr(om, _dp))
+        return px, py, cc, d2, om
+
+
+class CpuScene:
+    def __init__(self, r: _CpuRenderer, sd):
+        self.r = r
+        self.sd = sd
+        self._desc = sd.desc()
+        self.h = r._f("scene_create")(C.byref(self._desc))
+        if not self.h:
+            raise RuntimeError("scene_create failed")
+
+    def close(self):
+        if self.h:
+            self.r._f("scene_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def bvh_nodes(self):
+        return int(self.r._f("scene_bvh_nodes")(self.h))
+
+    @property
+    def total_power(self):
+        return float(self.r._f("scene_total_power")(self.h))
+
+    def closest_hit(self, rays, tmin=1e-6, stats=False):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = len(rays)
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        nv = np.zeros(n, np.int64)
+        lt = np.zeros(n, np.int64)
+        self.r._f("closest_hit")(self.h, _ptr(rays, _dp), n, tmin, _ptr(prim, _i32p), _ptr(t, _dp),
+                                 _ptr(nv, _i64p), _ptr(lt, _i64p))
+        return (prim, t, nv, lt) if stats else (prim, t)
+
+    def closest_hit_linear(self, rays, tmin=1e-6):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = len(rays)
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        self.r._f("closest_hit_linear")(self.h, _ptr(rays, _dp), n, tmin, _ptr(prim, _i32p), _ptr(t, _dp))
+        return prim, t
+
+    def occluded(self, rays, max_dist):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        md = np.ascontiguousarray(max_dist, np.float64)
+        occ = np.zeros(len(rays), np.uint8)
+        self.r._f("occluded")(self.h, _ptr(rays, _dp), _ptr(md, _dp), len(rays), _ptr(occ, _u8p))
+        return occ.astype(bool)
+
+
+class Oracle(_CpuRenderer):
+    prefix = "or_"
+
+    def __init__(self, path: str = ORACLE_SO):
+        super().__init__(path)
+        L = self.lib
+        L.or_render.argtypes = [C.c_void_p, C.POINTER(abi.CameraDesc), C.c_int64, C.c_uint64,
+                                C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                C.POINTER(abi.Energy), _u64p]
+        L.or_closest_hit_margin.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, C.c_double,
+                                            _i32p, _dp, _dp, _u8p]
+
+    def render(self, scene: CpuScene, cam, n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4,
+               max_bounce=32, rr_depth=0, threads=1):
+        npx = cam.res_x * cam.res_y
+        xyz = np.zeros((npx, 3))
+        hits = np.zeros(npx)
+        e = abi.Energy()
+        st = np.zeros(3, np.uint64)
+        self.lib.or_render(scene.h, C.byref(cam), n, seed_base, salt, hero_c, max_bounce, rr_depth,
+                           threads, _ptr(xyz, _dp), _ptr(hits, _dp), C.byref(e), _ptr(st, _u64p))
+        return xyz, hits, e.as_dict(), {"segments": int(st[0]), "connections": int(st[1]),
+                                       "splats": int(st[2])}
+
+    def closest_hit_margin(self, scene: CpuScene, rays, tmin=1e-6, rel_margin=1e-5):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = len(rays)
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        t2 = np.zeros(n)
+        edge = np.zeros(n, np.uint8)
+        self.lib.or_closest_hit_margin(scene.h, _ptr(rays, _dp), n, tmin, rel_margin,
+                                       _ptr(prim, _i32p), _ptr(t, _dp), _ptr(t2, _dp), _ptr(edge, _u8p))
+        return prim, t, t2, edge.astype(bool)
+
+
+class Reference(_CpuRenderer):
+    prefix = "ref_"
+
+    def __init__(self, path: str = REF_SO):
+        super().__init__(path)
+        self.lib.ref_render.argtypes = [C.c_void_p, C.POINTER(abi.CameraDesc), C.c_int64,
+                                        C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, _dp,
+                                        _dp, C.POINTER(abi.Energy), _u64p]
+
+    def render(self, scene: CpuScene, cam, n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4,
+               max_bounce=32, threads=1):
+        npx = cam.res_x * cam.res_y
+        xyz = np.zeros((npx, 3))
+        hits = np.zeros(npx)
+        e = abi.Energy()
+        rq = C.c_uint64(0)
+        self.lib.ref_render(scene.h, C.byref(cam), n, seed_base, salt, hero_c, max_bounce, threads,
+                            _ptr(xyz, _dp), _ptr(hits, _dp), C.byref(e), C.byref(rq))
+        return xyz, hits, e.as_dict(), {"ray_queries": int(rq.value)}
+
+
+def available(which: str = "oracle") -> bool:
+    return os.path.exists(ORACLE_SO if which == "oracle" else REF_SO)

@@ -1,0 +1,55 @@
+// Test infrastructure only: declarations of the oracle's C ABI.
+//
+// The oracle is a CPU restatement, in double precision, of the reference's mode-B
+// forward light tracer for the triangle/quad-light/pinhole subset the GPU core
+// supports. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+// use it, and only as the checker; the product never links or calls it.
+//
+// Pinning (how each part is checked, see tests/test_oracle_*.py):
+//   rng, spectra, CIE, emission sampler, hero bundle, cosine/glossy lobes,
+//   camera basis/projection, BVH closest/any hit, full films + energy ledgers:
+//     bit-exact against golden vectors written by the reference itself
+//     (oracle/_ref, tests/golden/make_golden.py), plus closed forms
+//     (PCG32/SplitMix64 published constants, N-BK7 n_d, Fresnel at normal
+//     incidence, energy ledger closure, brute-force closest hit).
+//   rr_depth > 0 and the closest-hit ambiguity margin are this repository's own
+//     extensions with no reference counterpart: rr_depth is pinned by unbiasedness
+//     (same expectation as rr_depth = 0) and the ledger; the margin by brute force.
+#pragma once
+#include <cstdint>
+#include "../include/goodphotons.h"
+
+extern "C" {
+void* or_scene_create(const gp_scene_desc* d);
+void or_scene_destroy(void* s);
+int64_t or_scene_bvh_nodes(void* s);
+double or_scene_total_power(void* s);
+
+int or_render(void* s, const gp_camera_desc* cam, int64_t N, uint64_t seedBase, uint64_t salt,
+              int heroC, int maxBounce, int rrDepth, int nThreads, double* xyz, double* hits,
+              gp_energy* eOut, uint64_t* stats3);
+
+int or_closest_hit(void* s, const double* rays, int64_t n, double tmin, int32_t* prim,
+                   double* t, int64_t* nodeVisits, int64_t* leafTests);
+int or_closest_hit_linear(void* s, const double* rays, int64_t n, double tmin, int32_t* prim,
+                          double* t);
+int or_closest_hit_margin(void* s, const double* rays, int64_t n, double tmin, double relMargin,
+                          int32_t* prim, double* t, double* t2, uint8_t* edge);
+int or_occluded(void* s, const double* rays, const double* maxDist, int64_t n, uint8_t* occ);
+
+void or_pcg_stream(uint64_t seq, uint64_t seed, int n, uint32_t* out);
+void or_seed_unit_uniforms(uint64_t k, uint64_t salt, int n, double* out);
+uint64_t or_mix64(uint64_t x);
+void or_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out);
+void or_cie(const double* lam, int n, double* xyz);
+double or_cie_y_integral();
+void or_emission_sample(const gp_spectrum* s, const double* u, int n, double* lam, double* pdf,
+                        double* integral);
+int or_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double* pdf);
+void or_cosine_hemisphere(const double* nrm, uint64_t seq, uint64_t seed, int n, double* out);
+void or_sample_glossy(const double* mdir, double rough, uint64_t seq, uint64_t seed, int n,
+                      double* out);
+void or_camera_basis(const gp_camera_desc* cd, double* out14);
+int or_camera_project(const gp_camera_desc* cd, const double* pts, int n, int32_t* px,
+                      int32_t* py, double* cosCam, double* dist2, double* omega);
+}
