@@ -60,7 +60,13 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(sd, target_s=12.0):
+def cpu_baseline(sd, target_s=10.0):
+    """Time the CPU forward path on a bounded sample of the same workload.
+
+    With the reference build present (oracle/_ref), the reference itself is timed
+    (kind "reference"); the oracle traces the same photons to count segments exactly
+    (its ray-query total must equal the reference's, a live bit-identity check).
+    Otherwise the oracle is timed (kind "port")."""
     import oracle
     if not oracle.available("oracle"):
         return None
@@ -68,29 +74,45 @@ def cpu_baseline(sd, target_s=12.0):
     cs = orc.scene(sd)
     T = cpu_threads()
     cam = sd.camera
-    pilot = 1 << 15
+    kw = dict(hero_c=sd.hero_c, max_bounce=sd.max_depth)
+    ref = oracle.Reference() if oracle.available("ref") and sd.rr_depth == 0 else None
+    timer = ref if ref is not None else orc
+    ts = timer.scene(sd) if ref is not None else cs
+    pilot = 1 << 16
     t0 = time.perf_counter()
-    orc.render(cs, cam, pilot, hero_c=sd.hero_c, max_bounce=sd.max_depth, rr_depth=sd.rr_depth, threads=T)
+    timer.render(ts, cam, pilot, seed_base=1 << 41, threads=T, **kw)
     dt = time.perf_counter() - t0
-    n = int(min(1 << 24, max(1 << 16, pilot * target_s / max(dt, 1e-3))))
+    n = int(min(1 << 25, max(1 << 16, pilot * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
-    _, _, _, st = orc.render(cs, cam, n, seed_base=1 << 40, hero_c=sd.hero_c, max_bounce=sd.max_depth,
-                             rr_depth=sd.rr_depth, threads=T)
+    _, _, _, tst = timer.render(ts, cam, n, seed_base=1 << 40, threads=T, **kw)
     dt = time.perf_counter() - t0
-    out = {"value": st["segments"] / dt, "unit": "segments/s", "cores": T, "kind": "port",
-           "sample": f"{n} photons of the same workload (512x512, depth 16, hero C=4), "
+    _, _, _, st = orc.render(cs, cam, n, seed_base=1 << 40, threads=T, rr_depth=sd.rr_depth, **kw)
+    out = {"value": st["segments"] / dt, "unit": "segments/s", "cores": T,
+           "kind": "reference" if ref is not None else "port",
+           "sample": f"{n} photons of the same workload (512x512, depth {sd.max_depth}, hero C={sd.hero_c}), "
                      f"{st['segments']} segments in {dt:.2f} s on {T} threads"}
-    if oracle.available("ref"):
-        ref = oracle.Reference()
-        rs = ref.scene(sd)
-        t0 = time.perf_counter()
-        _, _, _, rq = ref.render(rs, cam, n, seed_base=1 << 40, hero_c=sd.hero_c, max_bounce=sd.max_depth,
-                                 threads=T)
-        dt_ref = time.perf_counter() - t0
-        out["reference_seconds_same_sample"] = dt_ref
-        out["reference_ray_queries"] = rq["ray_queries"]
+    if ref is not None:
+        out["reference_ray_queries"] = tst["ray_queries"]
         out["port_ray_queries"] = st["segments"] + st["connections"]
+        out["bit_identical_ray_queries"] = tst["ray_queries"] == st["segments"] + st["connections"]
     return out
+
+
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (the runtime's own
+    copy kernel): the reference point that scales rocprof FETCH_SIZE/WRITE_SIZE."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def main():
@@ -101,7 +123,13 @@ def main():
     ap.add_argument("--photons", type=int, default=1 << 26)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (photon-range sharding "
               "plus one int64 film all-reduce is designed but not built); run with --gpus 1",

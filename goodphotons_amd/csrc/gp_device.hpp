@@ -253,17 +253,18 @@ __device__ __forceinline__ PairHit slabPair(const RayPre& r, float4 a, float4 b,
 // Traversal stack: the top `SD` entries live in LDS at [slot * stride + lane], deeper
 // entries spill to a per-thread global area. The host guarantees SD + spill depth
 // >= the tree's depth, and each level pushes at most one entry, so it never overflows.
-template <int SD>
+// WITH_T = false keeps node indices only (any-hit needs no entry distances).
+template <int SD, bool WITH_T = true>
 struct Stack {
     int* sidx;            // LDS, SD * stride ints
-    float* st;            // LDS, SD * stride floats
+    float* st;            // LDS, SD * stride floats (unused without WITH_T)
     int stride;
     int2* spill;          // global, spillDepth entries for this thread (may be null)
     int sp;
     __device__ __forceinline__ void push(int node, float t) {
         if (sp < SD) {
             sidx[sp * stride] = node;
-            st[sp * stride] = t;
+            if (WITH_T) st[sp * stride] = t;
         } else {
             spill[sp - SD] = make_int2(node, __float_as_int(t));
         }
@@ -273,7 +274,7 @@ struct Stack {
         --sp;
         if (sp < SD) {
             node = sidx[sp * stride];
-            t = st[sp * stride];
+            t = WITH_T ? st[sp * stride] : 0.0f;
         } else {
             int2 v = spill[sp - SD];
             node = v.x;
@@ -330,10 +331,10 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
 }
 
 // Any hit on [tmin, tmax).
-template <int SD>
+template <int SD, bool WT>
 __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                                          const float4* __restrict__ tris, const RayPre& r,
-                                         float tmin, float tmax, Stack<SD>& stk) {
+                                         float tmin, float tmax, Stack<SD, WT>& stk) {
     int node = 0;
     stk.sp = 0;
     while (true) {

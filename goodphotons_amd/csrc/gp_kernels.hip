@@ -454,15 +454,41 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ connect
+// Splat cache: a per-workgroup open-addressing table in LDS keyed by (job, pixel).
+// Unoccluded connections add their integer X, Y, Z and hit count into it with LDS
+// 64-bit atomics; the table is flushed to the film once per workgroup. Pixels that
+// many connections hit (the emitter's direct view, caustic foci) then cost one
+// global atomic per workgroup instead of one per connection. Integer sums are
+// order-independent, so the film is bit-identical to splatting directly.
+constexpr int kSplatSlots = 256;
+constexpr unsigned kSplatEmpty = 0xffffffffu;
+constexpr int kSplatPixelBits = 22;   // pixels < 4M and jobs < 1024 use the cache
+
+__device__ __forceinline__ void splatGlobal(long long* film, int pix, unsigned long long X,
+                                            unsigned long long Y, unsigned long long Z,
+                                            unsigned long long H) {
+    unsigned long long* f = (unsigned long long*)(film + 4 * (size_t)pix);
+    if (X) atomicAdd(f + 0, X);
+    if (Y) atomicAdd(f + 1, Y);
+    if (Z) atomicAdd(f + 2, Z);
+    atomicAdd(f + 3, H);
+}
+
 template <int SD>
 __global__ void __launch_bounds__(256) k_connect(Queues q) {
     __shared__ int sIdx[SD * 256];
-    __shared__ float sT[SD * 256];
+    __shared__ unsigned sKey[kSplatSlots];
+    __shared__ unsigned long long sVal[kSplatSlots * 4];
+    for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
+        sKey[k] = kSplatEmpty;
+        sVal[4 * k + 0] = 0; sVal[4 * k + 1] = 0; sVal[4 * k + 2] = 0; sVal[4 * k + 3] = 0;
+    }
+    __syncthreads();
     const unsigned n = q.ctrl->nConn;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
-                  q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    Stack<SD, false> stk{sIdx + threadIdx.x, nullptr, (int)blockDim.x,
+                         q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     JobAcc<2> cacc;   // connections, splats -> counters[2], [3]
     CounterAddr ca{q.jobs, 2};
     for (unsigned i = gtid; i < n; i += stride) {
@@ -473,21 +499,44 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
         accumulate(cacc, j, ca);
         cacc.v[0] += 1;
         RayPre r = prepareRay(xyz(co), xyz(cd));
-        const bool occ = traceAny<SD>(J.nodes, J.tris, r, kRayEps, co.w, stk);
+        const bool occ = traceAny<SD, false>(J.nodes, J.tris, r, kRayEps, co.w, stk);
         if (!occ) {
             cacc.v[1] += 1;
             const int pix = __float_as_int(cd.w);
-            unsigned long long* f = (unsigned long long*)(J.film + 4 * (size_t)pix);
             const unsigned long long X = min(toFixed((double)cx.x, J.filmScale), J.filmQmax);
             const unsigned long long Y = min(toFixed((double)cx.y, J.filmScale), J.filmQmax);
             const unsigned long long Z = min(toFixed((double)cx.z, J.filmScale), J.filmQmax);
-            if (X) atomicAdd(f + 0, X);
-            if (Y) atomicAdd(f + 1, Y);
-            if (Z) atomicAdd(f + 2, Z);
-            atomicAdd(f + 3, (unsigned long long)(tag >> 16));
+            const unsigned long long H = (unsigned long long)(tag >> 16);
+            int slot = -1;
+            if (pix < (1 << kSplatPixelBits) && j < (1 << (32 - kSplatPixelBits))) {
+                const unsigned key = ((unsigned)j << kSplatPixelBits) | (unsigned)pix;
+                const unsigned h = (key * 2654435761u) >> 24;
+                for (int probe = 0; probe < 4; ++probe) {
+                    const int s = (int)((h + probe) & (kSplatSlots - 1));
+                    unsigned cur = sKey[s];
+                    if (cur == kSplatEmpty) cur = atomicCAS(&sKey[s], kSplatEmpty, key);
+                    if (cur == kSplatEmpty || cur == key) { slot = s; break; }
+                }
+            }
+            if (slot >= 0) {
+                if (X) atomicAdd(&sVal[4 * slot + 0], X);
+                if (Y) atomicAdd(&sVal[4 * slot + 1], Y);
+                if (Z) atomicAdd(&sVal[4 * slot + 2], Z);
+                atomicAdd(&sVal[4 * slot + 3], H);
+            } else {
+                splatGlobal(J.film, pix, X, Y, Z, H);
+            }
         }
     }
     flushWave(cacc, ca);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
+        const unsigned key = sKey[k];
+        if (key == kSplatEmpty) continue;
+        const int j = (int)(key >> kSplatPixelBits);
+        const int pix = (int)(key & ((1u << kSplatPixelBits) - 1));
+        splatGlobal(q.jobs[j].film, pix, sVal[4 * k + 0], sVal[4 * k + 1], sVal[4 * k + 2], sVal[4 * k + 3]);
+    }
 }
 
 // ------------------------------------------------------------------ probes
@@ -521,12 +570,12 @@ __global__ void __launch_bounds__(256) k_probe_any(const float4* nodes, const fl
     __shared__ float sT[SD * 256];
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
-                  spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
+    Stack<SD, false> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
+                         spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
     for (long long i = gtid; i < n; i += stride) {
         const float* rp = rays + 6 * i;
         RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
-        occ[i] = traceAny<SD>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
+        occ[i] = traceAny<SD, false>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
     }
 }
 
