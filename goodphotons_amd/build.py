@@ -61,7 +61,24 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_demo()
     return LIB
+
+
+DEMO = os.path.join(PKG, "forward_demo")
+
+
+def build_demo() -> str:
+    """The C++ caller of host/gp_forward.hpp, linked against the library."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(PKG, "host"), os.path.join(PKG, "host", "forward_demo.cpp"),
+           "-o", DEMO + ".tmp", "-L", PKG, "-l:libgoodphotons.so", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DEMO + ".tmp", DEMO)
+    return DEMO
 
 
 if __name__ == "__main__":

@@ -37,46 +37,47 @@ __device__ __forceinline__ unsigned long long shflXor64(unsigned long long v, in
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Per-lane accumulators for one job at a time; flushed with a segmented wave
-// reduction (one atomic per distinct job in the wave and quantity).
+// Per-job sums (ledger terms, counters) without contended global atomics: each lane
+// keeps a register sum for the job it is currently on; when its job changes the sum
+// goes into a per-workgroup LDS table (one row per job slot), and the table is added
+// to global memory once per workgroup. A queue holding one job never touches the
+// table until the end; a mixed queue pays LDS atomics, not global ones.
 template <int K>
 struct JobAcc {
     int job = -1;
     unsigned long long v[K] = {};
 };
 
-template <int K, class Addr>
-__device__ __forceinline__ void flushLane(JobAcc<K>& a, Addr addr) {
+template <int K>
+__device__ __forceinline__ void initJobTable(unsigned long long* tab, int nJobs) {
+    for (int r = threadIdx.x; r < nJobs * K; r += blockDim.x) tab[r] = 0ull;
+    __syncthreads();
+}
+
+template <int K>
+__device__ __forceinline__ void spill(JobAcc<K>& a, unsigned long long* tab) {
     if (a.job >= 0)
         for (int k = 0; k < K; ++k)
-            if (a.v[k]) atomicAdd(addr(a.job, k), a.v[k]);
+            if (a.v[k]) atomicAdd(&tab[a.job * K + k], a.v[k]);
     for (int k = 0; k < K; ++k) a.v[k] = 0;
 }
 
-template <int K, class Addr>
-__device__ __forceinline__ void accumulate(JobAcc<K>& a, int job, Addr addr) {
+template <int K>
+__device__ __forceinline__ void accumulate(JobAcc<K>& a, int job, unsigned long long* tab) {
     if (a.job != job) {
-        flushLane(a, addr);
+        spill(a, tab);
         a.job = job;
     }
 }
 
-// Called by every lane of the wave (converged).
+// Called by every thread of the workgroup (converged).
 template <int K, class Addr>
-__device__ __forceinline__ void flushWave(JobAcc<K>& a, Addr addr) {
-    unsigned long long remaining = __ballot(a.job >= 0);
-    const unsigned lane = __lane_id();
-    while (remaining) {
-        int leader = __ffsll((long long)remaining) - 1;
-        int lj = __shfl(a.job, leader);
-        bool mine = (a.job == lj) && ((remaining >> lane) & 1ull);
-        unsigned long long mask = __ballot(mine);
-        for (int k = 0; k < K; ++k) {
-            unsigned long long s = mine ? a.v[k] : 0ull;
-            for (int m = 32; m >= 1; m >>= 1) s += shflXor64(s, m);
-            if ((int)lane == leader && s) atomicAdd(addr(lj, k), s);
-        }
-        remaining &= ~mask;
+__device__ __forceinline__ void flushJobTable(JobAcc<K>& a, unsigned long long* tab, int nJobs, Addr addr) {
+    spill(a, tab);
+    __syncthreads();
+    for (int r = threadIdx.x; r < nJobs * K; r += blockDim.x) {
+        const unsigned long long v = tab[r];
+        if (v) atomicAdd(addr(r / K, r % K), v);
     }
 }
 
@@ -137,36 +138,57 @@ __device__ __forceinline__ bool makeConnection(const DevJob& J, int job, F3 p, F
 }  // namespace
 
 // ------------------------------------------------------------------ schedule
-__global__ void k_schedule(Queues q, int cur) {
-    if (threadIdx.x != 0) return;
+// One workgroup of kMaxJobs threads: thread k owns position k of the priority order.
+// Photons still to emit per job are scanned (inclusive prefix in LDS) and each job
+// takes what fits in the free queue slots after the jobs ahead of it.
+__global__ void __launch_bounds__(kMaxJobs) k_schedule(Queues q, int cur) {
+    __shared__ unsigned sScan[kMaxJobs];
     Ctrl* c = q.ctrl;
+    const int k = threadIdx.x;
+    const int n = c->nOrder;
     const unsigned it = c->iter;
     const unsigned alive = c->nPaths[cur];
-    unsigned freeSlots = c->capacity > alive ? c->capacity - alive : 0u;
-    unsigned total = 0;
-    const int n = c->nOrder;
-    for (int k = 0; k < n; ++k) {
-        const int j = c->order[k];
-        c->emitPrefix[k] = total;
+    const unsigned freeSlots = c->capacity > alive ? c->capacity - alive : 0u;
+    int j = -1;
+    unsigned rem = 0;
+    if (k < n) {
+        j = c->order[k];
+        const unsigned long long nx = c->nextPhoton[j], en = c->endPhoton[j];
+        const unsigned long long r = (c->active[j] && en > nx) ? en - nx : 0ull;
+        rem = r > (unsigned long long)freeSlots ? freeSlots : (unsigned)r;
+    }
+    sScan[k] = rem;
+    __syncthreads();
+    for (int off = 1; off < kMaxJobs; off <<= 1) {
+        unsigned v = (k >= off) ? sScan[k - off] : 0u;
+        __syncthreads();
+        if (k >= off) sScan[k] = min(sScan[k] + v, 0x7fffffffu);
+        __syncthreads();
+    }
+    const unsigned before = (k > 0) ? sScan[k - 1] : 0u;
+    if (k < n) {
+        const unsigned room = freeSlots > before ? freeSlots - before : 0u;
+        const unsigned take = rem < room ? rem : room;
+        const unsigned start = before < freeSlots ? before : freeSlots;
+        c->emitPrefix[k] = start;
         c->emitJob[k] = j;
         c->emitStart[k] = c->nextPhoton[j];
-        unsigned long long rem = (c->active[j] && c->endPhoton[j] > c->nextPhoton[j])
-                                     ? c->endPhoton[j] - c->nextPhoton[j] : 0ull;
-        unsigned room = freeSlots - total;
-        unsigned take = rem < (unsigned long long)room ? (unsigned)rem : room;
         if (take) {
             c->nextPhoton[j] += take;
             if (c->nextPhoton[j] == c->endPhoton[j]) c->lastEmitIter[j] = (int)it;
         }
-        total += take;
     }
-    c->emitPrefix[n] = total;
-    c->nEmit = total;
-    c->emitBase = alive;
-    c->nPaths[cur] = alive + total;
-    c->nPaths[cur ^ 1] = 0;
-    c->nConn = 0;
-    c->iter = it + 1;
+    __syncthreads();
+    if (k == 0) {
+        const unsigned total = (n > 0) ? min(sScan[n - 1], freeSlots) : 0u;
+        c->emitPrefix[n] = total;
+        c->nEmit = total;
+        c->emitBase = alive;
+        c->nPaths[cur] = alive + total;
+        c->nPaths[cur ^ 1] = 0;
+        c->nConn = 0;
+        c->iter = it + 1;
+    }
 }
 
 // ------------------------------------------------------------------ emit
@@ -175,6 +197,10 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
     const unsigned nEmit = c->nEmit;
     const unsigned base = c->emitBase;
     const int nOrder = c->nOrder;
+    const int nJobs = c->nJobs;
+    __shared__ unsigned long long eTab[kMaxJobs], pTab[kMaxJobs];
+    initJobTable<1>(eTab, nJobs);
+    initJobTable<1>(pTab, nJobs);
     JobAcc<1> eacc;   // emitted
     JobAcc<1> pacc;   // photons
     EnergyAddr ea{q.jobs};
@@ -227,8 +253,8 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         }
         const float b = J.basePower / (float)C;
         for (int i = 0; i < C; ++i) beta[i] = b;
-        accumulate(eacc, j, ea);
-        accumulate(pacc, j, pa);
+        accumulate(eacc, j, eTab);
+        accumulate(pacc, j, pTab);
         pacc.v[0] += 1;
         float4 co, cd, cx;
         bool conn = false;
@@ -252,8 +278,8 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         unsigned meta = packMeta((unsigned)j, 0u, (unsigned)C) | (dead ? 0x80000000u : 0u);
         q.st[cur][slot] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), (unsigned)idx, meta);
     }
-    flushWave(eacc, ea);
-    flushWave(pacc, pa);
+    flushJobTable(eacc, eTab, nJobs, ea);
+    flushJobTable(pacc, pTab, nJobs, pa);
 }
 
 // ------------------------------------------------------------------ extend
@@ -266,6 +292,9 @@ __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
                   q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    const int nJobs = q.ctrl->nJobs;
+    __shared__ unsigned long long sTab[kMaxJobs];
+    initJobTable<1>(sTab, nJobs);
     JobAcc<1> sacc;
     CounterAddr sa{q.jobs, 1};
     for (unsigned i = gtid; i < n; i += stride) {
@@ -281,12 +310,12 @@ __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
             float tmax = INFINITY;
             best = traceClosest<SD>(J.nodes, J.tris, r, kRayEps, tmax, stk);
             t = tmax;
-            accumulate(sacc, j, sa);
+            accumulate(sacc, j, sTab);
             sacc.v[0] += 1;
         }
         q.hit[i] = make_float2(t, __int_as_float(best));
     }
-    flushWave(sacc, sa);
+    flushJobTable(sacc, sTab, nJobs, sa);
 }
 
 // ------------------------------------------------------------------ shade
@@ -294,6 +323,9 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     const int nxt = cur ^ 1;
     const unsigned n = q.ctrl->nPaths[cur];
     const unsigned stride = gridDim.x * blockDim.x;
+    const int nJobs = q.ctrl->nJobs;
+    __shared__ unsigned long long eTab[kMaxJobs * 3];
+    initJobTable<3>(eTab, nJobs);
     JobAcc<3> eacc;   // absorbed, escaped, residual -> energy[1], [3], [4]
     EnergyAddr ea{q.jobs};
     auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
@@ -311,7 +343,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             const int bounce = (int)((meta >> 16) & 0xffu);
             int nUp = (int)((meta >> 24) & 7u);
             const DevJob& J = q.jobs[j];
-            accumulate(eacc, j, eaddr);
+            accumulate(eacc, j, eTab);
             const float4 l4 = q.lam[cur][i], b4 = q.beta[cur][i];
             lam[0] = l4.x; lam[1] = l4.y; lam[2] = l4.z; lam[3] = l4.w;
             beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z; beta[3] = b4.w;
@@ -450,7 +482,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             q.st[nxt][pi] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), s4.z, meta);
         }
     }
-    flushWave(eacc, eaddr);
+    flushJobTable(eacc, eTab, nJobs, eaddr);
 }
 
 // ------------------------------------------------------------------ connect
@@ -489,6 +521,9 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD, false> stk{sIdx + threadIdx.x, nullptr, (int)blockDim.x,
                          q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    const int nJobs = q.ctrl->nJobs;
+    __shared__ unsigned long long cTab[kMaxJobs * 2];
+    initJobTable<2>(cTab, nJobs);
     JobAcc<2> cacc;   // connections, splats -> counters[2], [3]
     CounterAddr ca{q.jobs, 2};
     for (unsigned i = gtid; i < n; i += stride) {
@@ -496,7 +531,7 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
         const unsigned tag = (unsigned)__float_as_int(cx.w);
         const int j = (int)(tag & 0xffffu);
         const DevJob& J = q.jobs[j];
-        accumulate(cacc, j, ca);
+        accumulate(cacc, j, cTab);
         cacc.v[0] += 1;
         RayPre r = prepareRay(xyz(co), xyz(cd));
         const bool occ = traceAny<SD, false>(J.nodes, J.tris, r, kRayEps, co.w, stk);
@@ -528,7 +563,7 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
             }
         }
     }
-    flushWave(cacc, ca);
+    flushJobTable(cacc, cTab, nJobs, ca);
     __syncthreads();
     for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
         const unsigned key = sKey[k];
@@ -586,7 +621,7 @@ __global__ void k_fill64(unsigned long long* p, size_t n, unsigned long long v) 
 
 // ------------------------------------------------------------------ launchers
 void launchSchedule(const Queues& q, int cur, hipStream_t s) {
-    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(64), 0, s, q, cur);
+    hipLaunchKernelGGL(k_schedule, dim3(1), dim3(kMaxJobs), 0, s, q, cur);
 }
 void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
     hipLaunchKernelGGL(k_emit, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);

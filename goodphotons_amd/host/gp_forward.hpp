@@ -1,0 +1,237 @@
+// C++20 host entry point of the forward light-tracing core, over the C ABI
+// (include/goodphotons.h). It keeps the contract of the reference renderer's
+// GPU forward mode (src/render_cuda.h):
+//
+//   Film renderForwardGpu(scene, cam, resX, resY, N, eOut, diffraction, camMode,
+//                         seedBase = 0, wavefront = false, heroC = 1)
+//       <- renderForwardCuda (src/render_cuda.h:79-82): a NEW film per call (XYZ
+//          un-normalised sums + per-pixel hit counts), eOut ACCUMULATED (+=),
+//          seedBase = photons already traced, an EMPTY (zeroed, allocated) film when
+//          the scene or camera model is unsupported or there is no device.
+//   gpuAvailable / gpuDeviceName / gpuMemInfo / gpuForwardSupported
+//       <- cudaAvailable / cudaDeviceName / cudaMemInfo / cudaForwardSupported
+//          (src/render_cuda.h:24-56)
+//   ForwardSession (begin / batch / download / end)
+//       <- sharedForwardGpuBegin / Batch / Download / End (src/render_cuda.h:121-143),
+//          one camera: scene and film resident in HBM between batches.
+//
+// The scene types mirror the subset of the reference's Scene the core supports
+// (triangles, Diffuse / Dielectric / Glossy materials with analytic spectra, quad
+// area lights, pinhole camera); INTEGRATION.md shows the adapter from the
+// reference's own Scene/Camera.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/goodphotons.h"
+
+namespace gp {
+
+struct Vec3d {
+    double x = 0, y = 0, z = 0;
+};
+
+struct Tri {
+    Vec3d v0, v1, v2;
+    int matId = 0;
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<gp_material> mats;
+    std::vector<gp_emitter> emitters;
+};
+
+struct Camera {
+    Vec3d eye, target, up{0, 1, 0};
+    double fovYDeg = 40.0;
+};
+
+struct Film {
+    int resX = 0, resY = 0;
+    std::vector<Vec3d> xyz;     // accumulated XYZ per pixel (un-normalised)
+    std::vector<double> hits;   // contributions per pixel
+    void alloc() {
+        xyz.assign((size_t)resX * resY, Vec3d{});
+        hits.assign((size_t)resX * resY, 0.0);
+    }
+};
+
+struct EnergyReport {
+    double emitted = 0, absorbed = 0, sensor = 0, escaped = 0, residual = 0;
+};
+
+inline uint64_t kReferenceSalt = 0x9E3779B97F4A7C15ULL;
+inline int gMaxDepth = 32;      // Renderer::maxBounce default
+inline int gRrDepth = 0;        // Russian roulette from the first vertex, as the reference
+
+namespace detail {
+struct SceneHandle {
+    gp_scene* s = nullptr;
+    ~SceneHandle() { if (s) gp_scene_destroy(s); }
+};
+struct FilmHandle {
+    gp_film* f = nullptr;
+    ~FilmHandle() { if (f) gp_film_destroy(f); }
+};
+inline gp_scene_desc describe(const Scene& sc, std::vector<float>& verts, std::vector<int32_t>& mats) {
+    verts.resize(sc.tris.size() * 9);
+    mats.resize(sc.tris.size());
+    for (size_t i = 0; i < sc.tris.size(); ++i) {
+        const Tri& t = sc.tris[i];
+        const double v[9] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x, t.v1.y, t.v1.z, t.v2.x, t.v2.y, t.v2.z};
+        for (int k = 0; k < 9; ++k) verts[9 * i + k] = (float)v[k];
+        mats[i] = t.matId;
+    }
+    gp_scene_desc d{};
+    d.n_tris = (int64_t)sc.tris.size();
+    d.n_materials = (int32_t)sc.mats.size();
+    d.n_emitters = (int32_t)sc.emitters.size();
+    d.verts = verts.data();
+    d.mat_ids = mats.data();
+    d.materials = sc.mats.data();
+    d.emitters = sc.emitters.data();
+    return d;
+}
+inline gp_camera_desc camera(const Camera& c, int resX, int resY) {
+    gp_camera_desc d{};
+    const Vec3d* v[3] = {&c.eye, &c.target, &c.up};
+    double* o[3] = {d.eye, d.target, d.up};
+    for (int k = 0; k < 3; ++k) { o[k][0] = v[k]->x; o[k][1] = v[k]->y; o[k][2] = v[k]->z; }
+    d.fov_y_deg = c.fovYDeg;
+    d.res_x = resX;
+    d.res_y = resY;
+    return d;
+}
+inline void report(const char* what) {
+    std::fprintf(stderr, "[goodphotons] %s: %s\n", what, gp_last_error());
+}
+inline bool download(gp_film* f, Film& out) {
+    std::vector<double> xyz((size_t)out.resX * out.resY * 3);
+    if (gp_film_download(f, xyz.data(), out.hits.data()) != GP_OK) return false;
+    for (size_t i = 0; i < out.xyz.size(); ++i) out.xyz[i] = Vec3d{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    return true;
+}
+}  // namespace detail
+
+inline bool gpuAvailable() {
+    gp_device_info i;
+    return gp_device_query(&i) == GP_OK;
+}
+inline std::string gpuDeviceName() {
+    gp_device_info i;
+    if (gp_device_query(&i) != GP_OK) return "none";
+    return std::string(i.name[0] ? i.name : i.arch);
+}
+inline bool gpuMemInfo(size_t* freeB, size_t* totalB) {
+    gp_device_info i;
+    if (gp_device_query(&i) != GP_OK) return false;
+    if (freeB) *freeB = (size_t)i.free_mem;
+    if (totalB) *totalB = (size_t)i.total_mem;
+    return true;
+}
+// The support gate (the reference's cudaForwardSupported): validates by building.
+inline bool gpuForwardSupported(const Scene& sc) {
+    std::vector<float> v;
+    std::vector<int32_t> m;
+    gp_scene_desc d = detail::describe(sc, v, m);
+    detail::SceneHandle h;
+    return gp_scene_create(&d, &h.s) == GP_OK;
+}
+
+inline Film renderForwardGpu(const Scene& scene, const Camera& cam, int resX, int resY, long long N,
+                             EnergyReport& eOut, bool diffraction, char camMode,
+                             unsigned long long seedBase = 0, bool wavefront = false, int heroC = 1) {
+    (void)diffraction;   // gratings are outside the supported material set
+    (void)wavefront;     // the core always runs the wavefront
+    Film film;
+    film.resX = resX;
+    film.resY = resY;
+    film.alloc();
+    if (camMode != 'B') {
+        std::fprintf(stderr, "[goodphotons] camera model %c not supported (pinhole 'B' only)\n", camMode);
+        return film;
+    }
+    std::vector<float> v;
+    std::vector<int32_t> m;
+    gp_scene_desc d = detail::describe(scene, v, m);
+    detail::SceneHandle sh;
+    if (gp_scene_create(&d, &sh.s) != GP_OK) { detail::report("scene"); return film; }
+    detail::FilmHandle fh;
+    if (gp_film_create(resX, resY, N > 0 ? N : 1, &fh.f) != GP_OK) { detail::report("film"); return film; }
+    gp_camera_desc cd = detail::camera(cam, resX, resY);
+    gp_trace_params p{};
+    p.n_photons = N;
+    p.seed_base = seedBase;
+    p.salt = kReferenceSalt;
+    p.hero_c = heroC < 1 ? 1 : (heroC > 4 ? 4 : heroC);
+    p.max_depth = gMaxDepth;
+    p.rr_depth = gRrDepth;
+    gp_energy e{};
+    if (gp_trace(sh.s, &cd, fh.f, &p, &e, nullptr) != GP_OK) { detail::report("trace"); return film; }
+    if (!detail::download(fh.f, film)) { detail::report("download"); film.alloc(); return film; }
+    eOut.emitted += e.emitted;
+    eOut.absorbed += e.absorbed;
+    eOut.sensor += e.sensor;
+    eOut.escaped += e.escaped;
+    eOut.residual += e.residual;
+    return film;
+}
+
+// Resident forward session (one camera): the scene and film stay in HBM; every
+// batch traces N more photons into the running totals.
+class ForwardSession {
+public:
+    bool begin(const Scene& scene, const Camera& cam, int resX, int resY, int heroC,
+               long long maxPhotons = 0) {
+        end();
+        std::vector<float> v;
+        std::vector<int32_t> m;
+        gp_scene_desc d = detail::describe(scene, v, m);
+        if (gp_scene_create(&d, &scene_) != GP_OK) { detail::report("scene"); return false; }
+        if (gp_film_create(resX, resY, maxPhotons, &film_) != GP_OK) { detail::report("film"); end(); return false; }
+        cam_ = detail::camera(cam, resX, resY);
+        heroC_ = heroC;
+        return true;
+    }
+    bool batch(long long N, unsigned long long seedBase) {
+        gp_trace_params p{};
+        p.n_photons = N;
+        p.seed_base = seedBase;
+        p.salt = kReferenceSalt;
+        p.hero_c = heroC_;
+        p.max_depth = gMaxDepth;
+        p.rr_depth = gRrDepth;
+        if (gp_trace(scene_, &cam_, film_, &p, &energy_, nullptr) != GP_OK) { detail::report("batch"); return false; }
+        return true;
+    }
+    // Running totals (REPLACES the caller's accumulators).
+    bool download(Film& film, EnergyReport& e) {
+        film.resX = cam_.res_x;
+        film.resY = cam_.res_y;
+        film.alloc();
+        if (!detail::download(film_, film)) return false;
+        e = EnergyReport{energy_.emitted, energy_.absorbed, energy_.sensor, energy_.escaped, energy_.residual};
+        return true;
+    }
+    void end() {
+        if (film_) gp_film_destroy(film_);
+        if (scene_) gp_scene_destroy(scene_);
+        film_ = nullptr;
+        scene_ = nullptr;
+        energy_ = gp_energy{};
+    }
+    ~ForwardSession() { end(); }
+
+private:
+    gp_scene* scene_ = nullptr;
+    gp_film* film_ = nullptr;
+    gp_camera_desc cam_{};
+    gp_energy energy_{};
+    int heroC_ = 4;
+};
+
+}  // namespace gp

@@ -1,0 +1,62 @@
+"""Full-size large-mesh case: the 5,242,880-triangle noise-displaced icosphere.
+Closest-hit prim ids at full scale against the oracle's BVH (every mismatch must
+be an ambiguous ray by the oracle's brute-force margin test), and a small-budget
+film against the oracle on the same photons."""
+import numpy as np
+import pytest
+
+import goodphotons_amd as gp
+from goodphotons_amd import scenes
+from tests.parity import image_metrics, seeded_rays
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2(gplib, orc):
+    sd = scenes.icosphere_c2(level=9, res=128)
+    print(f"[c2] tris={sd.n_tris}", flush=True)
+    gs = gp.GpuScene(sd, gplib)
+    cs = orc.scene(sd)
+    return sd, gs, cs
+
+
+def test_c2_prim_ids_full_scale(c2, orc):
+    sd, gs, cs = c2
+    info = gs.info()
+    print(f"[c2] bvh depth {info['bvh_depth']} nodes {info['n_nodes']} build {info['build_ms']:.0f} ms", flush=True)
+    # rays aimed at the shell from around it, so most of them hit the dense mesh
+    rng = np.random.default_rng(11)
+    n = 100000
+    o = np.array([0.5, 0.31, 0.5]) + rng.normal(size=(n, 3)) * 0.6
+    tgt = np.array([0.5, 0.31, 0.5]) + rng.normal(size=(n, 3)) * 0.15
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    gprim, gt = gs.closest_hit(rays)
+    cprim, ct = cs.closest_hit(rays.astype(np.float64))
+    diff = np.nonzero(gprim != cprim)[0]
+    print(f"[c2] hits {int((cprim >= 0).sum())} mismatches {len(diff)}", flush=True)
+    assert len(diff) <= n // 1000
+    if len(diff):
+        p, t, t2, edge = orc.closest_hit_margin(cs, rays[diff].astype(np.float64), 1e-6, 1e-5)
+        near_tie = np.isfinite(t2) & (np.abs(t2 - t) <= 1e-6 * np.maximum(np.abs(t), 1e-30))
+        amb = near_tie | edge | (~np.isfinite(t) & np.isfinite(t2) & (gprim[diff] >= 0))
+        assert np.all(amb), f"{int((~amb).sum())} unambiguous mismatches"
+    both = (gprim == cprim) & (cprim >= 0)
+    assert np.all(np.abs(gt[both] - ct[both]) <= 2e-6 * (1.0 + ct[both]))
+
+
+def test_c2_film_matches_oracle(c2, orc):
+    # Through 5 M small glass facets a few photons' float32 paths part from the double
+    # ones and become independent caustic samples; that difference is noise that
+    # shrinks with photons per pixel, so the film is compared at 2^20 photons on 64^2.
+    sd, gs, cs = c2
+    n = 1 << 20
+    cam = sd.with_camera_res(64).camera
+    gx, gh, ge, gst = gp.render_forward(gs, cam, n, hero_c=4, max_depth=12)
+    cx, ch, ce, cst = orc.render(cs, cam, n, hero_c=4, max_bounce=12, threads=8)
+    m = image_metrics(gx, cx, n, n, res=(64, 64))
+    print(f"[c2] film {m} segments gpu {gst['segments']} cpu {cst['segments']}", flush=True)
+    assert m["rel_rmse_max"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
+    assert abs(gst["segments"] / cst["segments"] - 1) < 5e-3
