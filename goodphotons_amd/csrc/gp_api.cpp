@@ -388,20 +388,17 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
         if ((st = fillJob(s, *cam, film, cp, 0, J)) != GP_OK) return st;
         E.hJobs[0] = J;
         Ctrl& C = *E.hCtrl;
-        std::memset(&C, 0, sizeof C);
+        E.initCtrl();
         C.nJobs = 1;
         C.nOrder = 1;
         C.order[0] = 0;
-        C.capacity = E.capacity;
-        C.connCapacity = 2 * E.capacity;
         C.nextPhoton[0] = 0;
         C.endPhoton[0] = (unsigned long long)n;
         C.active[0] = 1;
         C.lastEmitIter[0] = -1;
         GP_HIP(hipMemcpyAsync(E.dJobs, E.hJobs, sizeof(DevJob), hipMemcpyHostToDevice, strm));
         GP_HIP(hipMemcpyAsync(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, strm));
-        GP_HIP(hipMemsetAsync(E.dJobEnergy, 0, 5 * sizeof(unsigned long long), strm));
-        GP_HIP(hipMemsetAsync(E.dJobCounters, 0, 4 * sizeof(unsigned long long), strm));
+        if ((st = E.resetJobAcc(0)) != GP_OK) return st;
         E.parity = 0;
         const int K = std::max(2, E.tun.iters_per_graph & ~1);
         // Every iteration retires or advances each live path by one segment, so the
@@ -412,12 +409,13 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
             if ((st = E.runIterations(K)) != GP_OK) return st;
             iters += K;
             if ((st = E.readCtrl()) != GP_OK) return st;
-            if (C.nextPhoton[0] >= (unsigned long long)n && C.nPaths[E.parity] == 0) break;
+            unsigned live = 0;
+            for (int g = 0; g < kSegs; ++g) live += C.pathCount[E.parity][g][0];
+            if (C.nextPhoton[0] >= (unsigned long long)n && live == 0) break;
             if (iters > maxIters) return fail(GP_ERR_STATE, "wavefront did not drain");
         }
         unsigned long long ev[5], cv[4];
-        GP_HIP(hipMemcpy(ev, E.dJobEnergy, sizeof ev, hipMemcpyDeviceToHost));
-        GP_HIP(hipMemcpy(cv, E.dJobCounters, sizeof cv, hipMemcpyDeviceToHost));
+        if ((st = E.readJobAcc(0, ev, cv)) != GP_OK) return st;
         eacc.emitted += (double)ev[0] / J.energyScale;
         eacc.absorbed += (double)ev[1] / J.energyScale;
         eacc.sensor += (double)ev[2] / J.energyScale;

@@ -81,19 +81,57 @@ __device__ __forceinline__ void flushJobTable(JobAcc<K>& a, unsigned long long* 
     }
 }
 
-struct EnergyAddr {
+// Ledger and counter sums go to the job's row (blockIdx.x % kSegs): eight rows on
+// eight cache lines, so the per-workgroup flushes of a launch spread over eight
+// addresses instead of serialising on one.
+struct AccAddr {
     const DevJob* jobs;
+    int base;   // first index of this kernel's quantities in the row
     __device__ unsigned long long* operator()(int j, int k) const {
-        return (unsigned long long*)(jobs[j].energy + k);
+        return jobs[j].acc + (size_t)(blockIdx.x % kSegs) * kAccRow + base + k;
     }
 };
-struct CounterAddr {
-    const DevJob* jobs;
-    int base;
-    __device__ unsigned long long* operator()(int j, int k) const {
-        return jobs[j].counters + base + k;
+constexpr int kAccEnergy = 0;     // emitted, absorbed, sensor, escaped, residual
+constexpr int kAccPhotons = 5;    // then segments, connections, splats
+
+// Logical index -> slot of a segmented queue. `pre` is the prefix of the segment
+// counts (wave-uniform); the segment is the last one whose prefix is <= i, which
+// skips empty segments.
+struct SegMap {
+    unsigned pre[kSegs];
+    unsigned cap;
+    __device__ __forceinline__ unsigned slot(unsigned i) const {
+        unsigned base = 0, off = i;
+#pragma unroll
+        for (int g = 1; g < kSegs; ++g)
+            if (i >= pre[g]) { base = (unsigned)g * cap; off = i - pre[g]; }
+        return base + off;
     }
 };
+
+__device__ __forceinline__ unsigned loadSegMap(const unsigned (*cnt)[kLine], unsigned cap, SegMap& m) {
+    unsigned acc = 0;
+#pragma unroll
+    for (int g = 0; g < kSegs; ++g) {
+        m.pre[g] = acc;
+        acc += cnt[g][0];
+    }
+    m.cap = cap;
+    return acc;
+}
+
+// Append to this workgroup's segment (blockIdx.x % kSegs) of a segmented queue;
+// returns the slot, or ~0u (and raises the overflow flag) if the segment is full.
+__device__ __forceinline__ unsigned segAppend(unsigned (*cnt)[kLine], unsigned cap, bool pred,
+                                              unsigned* overflow) {
+    const unsigned g = blockIdx.x % kSegs;
+    const unsigned off = waveAppend(&cnt[g][0], pred);
+    if (pred && off >= cap) {
+        *overflow = 1u;
+        return ~0u;
+    }
+    return g * cap + off;
+}
 
 // Camera connection record for a diffuse (or emitter) vertex at p with shading
 // normal n (flat triangles: n is also the oriented geometric normal, so the
@@ -140,14 +178,19 @@ __device__ __forceinline__ bool makeConnection(const DevJob& J, int job, F3 p, F
 // ------------------------------------------------------------------ schedule
 // One workgroup of kMaxJobs threads: thread k owns position k of the priority order.
 // Photons still to emit per job are scanned (inclusive prefix in LDS) and each job
-// takes what fits in the free queue slots after the jobs ahead of it.
+// takes what fits in the free queue slots after the jobs ahead of it. The emitted
+// photons are then placed after the survivors of each segment, filling segments in
+// order; the segments' free room adds up to at least the free capacity, so the
+// number emitted depends only on the total number of survivors.
 __global__ void __launch_bounds__(kMaxJobs) k_schedule(Queues q, int cur) {
     __shared__ unsigned sScan[kMaxJobs];
     Ctrl* c = q.ctrl;
     const int k = threadIdx.x;
     const int n = c->nOrder;
     const unsigned it = c->iter;
-    const unsigned alive = c->nPaths[cur];
+    unsigned alive = 0;
+#pragma unroll
+    for (int g = 0; g < kSegs; ++g) alive += c->pathCount[cur][g][0];
     const unsigned freeSlots = c->capacity > alive ? c->capacity - alive : 0u;
     int j = -1;
     unsigned rem = 0;
@@ -183,28 +226,48 @@ __global__ void __launch_bounds__(kMaxJobs) k_schedule(Queues q, int cur) {
         const unsigned total = (n > 0) ? min(sScan[n - 1], freeSlots) : 0u;
         c->emitPrefix[n] = total;
         c->nEmit = total;
-        c->emitBase = alive;
-        c->nPaths[cur] = alive + total;
-        c->nPaths[cur ^ 1] = 0;
-        c->nConn = 0;
+        unsigned left = total, placed = 0;
+        for (int g = 0; g < kSegs; ++g) {
+            const unsigned cnt = c->pathCount[cur][g][0];
+            const unsigned room = c->segCap > cnt ? c->segCap - cnt : 0u;
+            const unsigned e = left < room ? left : room;
+            c->segBase[g] = cnt;
+            c->segEmit[g] = placed;
+            c->pathCount[cur][g][0] = cnt + e;
+            c->pathCount[cur ^ 1][g][0] = 0;
+            c->connCount[g][0] = 0;
+            placed += e;
+            left -= e;
+        }
+        c->segEmit[kSegs] = placed;
+        if (left) c->overflow = 1u;
         c->iter = it + 1;
     }
 }
 
 // ------------------------------------------------------------------ emit
 __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
-    const Ctrl* c = q.ctrl;
+    Ctrl* c = q.ctrl;
     const unsigned nEmit = c->nEmit;
-    const unsigned base = c->emitBase;
     const int nOrder = c->nOrder;
     const int nJobs = c->nJobs;
+    // emission slot s goes to segment g (segEmit[g] <= s < segEmit[g + 1]), after
+    // that segment's segBase[g] survivors
+    SegMap em;
+    unsigned emBase[kSegs];
+#pragma unroll
+    for (int g = 0; g < kSegs; ++g) {
+        em.pre[g] = c->segEmit[g];
+        emBase[g] = c->segBase[g];
+    }
+    em.cap = q.segCap;
     __shared__ unsigned long long eTab[kMaxJobs], pTab[kMaxJobs];
     initJobTable<1>(eTab, nJobs);
     initJobTable<1>(pTab, nJobs);
     JobAcc<1> eacc;   // emitted
     JobAcc<1> pacc;   // photons
-    EnergyAddr ea{q.jobs};
-    CounterAddr pa{q.jobs, 0};
+    AccAddr ea{q.jobs, kAccEnergy};
+    AccAddr pa{q.jobs, kAccPhotons};
     const unsigned stride = gridDim.x * blockDim.x;
     for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < nEmit; s += stride) {
         int k = 0;
@@ -263,14 +326,25 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
             const float one[kMaxHero] = {1.f, 1.f, 1.f, 1.f};
             conn = makeConnection(J, j, org, en, lam, beta, one, C, co, cd, cx);
         }
-        const unsigned ci = waveAppend(&q.ctrl->nConn, conn);
-        if (conn) {
+        const unsigned ci = segAppend(c->connCount, q.connSegCap, conn, &c->overflow);
+        if (conn && ci != ~0u) {
             q.co[ci] = co;
             q.cd[ci] = cd;
             q.cx[ci] = cx;
         }
         F3 o = org + dir * kRayEps;
-        const unsigned slot = base + s;
+        unsigned slot = 0;
+        {
+            unsigned off = s, g0 = 0;
+#pragma unroll
+            for (int g = 1; g < kSegs; ++g)
+                if (s >= em.pre[g]) { g0 = (unsigned)g; off = s - em.pre[g]; }
+            unsigned b = emBase[0];
+#pragma unroll
+            for (int g = 1; g < kSegs; ++g)
+                if (g0 == (unsigned)g) b = emBase[g];
+            slot = g0 * q.segCap + b + off;
+        }
         q.ro[cur][slot] = make_float4(o.x, o.y, o.z, 0.f);
         q.rd[cur][slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
         q.lam[cur][slot] = make_float4(lam[0], lam[1], lam[2], lam[3]);
@@ -287,7 +361,8 @@ template <int SD>
 __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
     __shared__ int sIdx[SD * 256];
     __shared__ float sT[SD * 256];
-    const unsigned n = q.ctrl->nPaths[cur];
+    SegMap sm;
+    const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
@@ -296,8 +371,9 @@ __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
     __shared__ unsigned long long sTab[kMaxJobs];
     initJobTable<1>(sTab, nJobs);
     JobAcc<1> sacc;
-    CounterAddr sa{q.jobs, 1};
-    for (unsigned i = gtid; i < n; i += stride) {
+    AccAddr sa{q.jobs, kAccPhotons + 1};
+    for (unsigned li = gtid; li < n; li += stride) {
+        const unsigned i = sm.slot(li);
         const uint4 s4 = q.st[cur][i];
         int best = -1;
         float t = INFINITY;
@@ -321,15 +397,18 @@ __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
 // ------------------------------------------------------------------ shade
 __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     const int nxt = cur ^ 1;
-    const unsigned n = q.ctrl->nPaths[cur];
+    Ctrl* ctl = q.ctrl;
+    SegMap sm;
+    const unsigned n = loadSegMap(ctl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
-    const int nJobs = q.ctrl->nJobs;
+    const int nJobs = ctl->nJobs;
     __shared__ unsigned long long eTab[kMaxJobs * 3];
     initJobTable<3>(eTab, nJobs);
-    JobAcc<3> eacc;   // absorbed, escaped, residual -> energy[1], [3], [4]
-    EnergyAddr ea{q.jobs};
+    JobAcc<3> eacc;   // absorbed, escaped, residual -> ledger entries 1, 3, 4
+    AccAddr ea{q.jobs, kAccEnergy};
     auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
+        const unsigned i = sm.slot(li);
         const uint4 s4 = q.st[cur][i];
         bool alive = false, conn = false;
         float4 co, cd, cx;
@@ -467,14 +546,14 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
                 meta = packMeta((unsigned)j, (unsigned)(bounce + 1), (unsigned)nUp);
             }
         }
-        const unsigned ci = waveAppend(&q.ctrl->nConn, conn);
-        if (conn) {
+        const unsigned ci = segAppend(ctl->connCount, q.connSegCap, conn, &ctl->overflow);
+        if (conn && ci != ~0u) {
             q.co[ci] = co;
             q.cd[ci] = cd;
             q.cx[ci] = cx;
         }
-        const unsigned pi = waveAppend(&q.ctrl->nPaths[nxt], alive);
-        if (alive) {
+        const unsigned pi = segAppend(ctl->pathCount[nxt], q.segCap, alive, &ctl->overflow);
+        if (alive && pi != ~0u) {
             q.ro[nxt][pi] = make_float4(no.x, no.y, no.z, 0.f);
             q.rd[nxt][pi] = make_float4(nd.x, nd.y, nd.z, 0.f);
             q.lam[nxt][pi] = make_float4(lam[0], lam[1], lam[2], lam[3]);
@@ -516,7 +595,8 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
         sVal[4 * k + 0] = 0; sVal[4 * k + 1] = 0; sVal[4 * k + 2] = 0; sVal[4 * k + 3] = 0;
     }
     __syncthreads();
-    const unsigned n = q.ctrl->nConn;
+    SegMap sm;
+    const unsigned n = loadSegMap(q.ctrl->connCount, q.connSegCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD, false> stk{sIdx + threadIdx.x, nullptr, (int)blockDim.x,
@@ -524,9 +604,10 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
     const int nJobs = q.ctrl->nJobs;
     __shared__ unsigned long long cTab[kMaxJobs * 2];
     initJobTable<2>(cTab, nJobs);
-    JobAcc<2> cacc;   // connections, splats -> counters[2], [3]
-    CounterAddr ca{q.jobs, 2};
-    for (unsigned i = gtid; i < n; i += stride) {
+    JobAcc<2> cacc;   // connections, splats
+    AccAddr ca{q.jobs, kAccPhotons + 2};
+    for (unsigned li = gtid; li < n; li += stride) {
+        const unsigned i = sm.slot(li);
         const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
         const unsigned tag = (unsigned)__float_as_int(cx.w);
         const int j = (int)(tag & 0xffffu);

@@ -7,7 +7,9 @@
 namespace gp {
 
 // Device pointers of one wavefront's queues. Path queues are double-buffered:
-// iteration k reads queue (k & 1) and writes survivors to the other.
+// iteration k reads queue (k & 1) and writes survivors to the other. Each queue is
+// kSegs segments of segCap (paths) / connSegCap (connections) slots; a logical index
+// maps to (segment, offset) through the prefix of the per-segment counts.
 struct Queues {
     float4* ro[2];
     float4* rd[2];
@@ -23,6 +25,7 @@ struct Queues {
     int2* spill;     // traversal stack spill, spillDepth entries per thread
     int spillDepth;
     int lds;         // LDS stack entries per lane (8, 16, 24 or 32)
+    unsigned segCap, connSegCap;
 };
 
 struct LaunchShape {

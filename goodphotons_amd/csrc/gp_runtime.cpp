@@ -245,10 +245,9 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
     J.film = (long long*)film->d;
     J.filmScale = film->scale;
     J.filmQmax = film->qmax;
-    J.energy = (long long*)(E.dJobEnergy + 5 * (size_t)slot);
+    J.acc = E.dJobAcc + (size_t)slot * kSegs * kAccRow;
     const double ebits = std::floor(60.0 - std::log2(std::max(1.0, (double)p.n_photons)) - 2.0);
     J.energyScale = std::ldexp(1.0, (int)std::max(8.0, ebits)) / std::max(s->basePower, 1e-300);
-    J.counters = E.dJobCounters + 4 * (size_t)slot;
     J.seedBase = p.seed_base;
     J.salt = p.salt;
     J.heroC = p.hero_c;
@@ -284,15 +283,15 @@ void Engine::releaseQueues() {
     if (dJobs) hipFree(dJobs);
     if (hCtrl) hipHostFree(hCtrl);
     if (hJobs) hipHostFree(hJobs);
-    if (dJobEnergy) hipFree(dJobEnergy);
-    if (dJobCounters) hipFree(dJobCounters);
+    if (dJobAcc) hipFree(dJobAcc);
     dCtrl = nullptr; dJobs = nullptr; hCtrl = nullptr; hJobs = nullptr;
-    dJobEnergy = nullptr; dJobCounters = nullptr;
+    dJobAcc = nullptr;
     int2* sp = q.spill;
     q = Queues{};
     q.spill = sp;
     q.spillDepth = spillDepth;
     capacity = 0;
+    segCap = 0;
 }
 
 void Engine::releaseSpill() {
@@ -309,6 +308,53 @@ void Engine::release() {
     releaseSpill();
 }
 
+// Largest number of items that the workgroups b with b % kSegs == g visit, over g,
+// when `blocks` workgroups of 256 threads grid-stride over cap items (the loop
+// shape of k_emit and k_shade). A segment receives at most one path and one
+// connection per item its workgroups visit, so this bounds every segment's fill.
+// The items sum to cap over the segments, so kSegs * bound >= cap.
+unsigned segmentBound(unsigned cap, unsigned blocks) {
+    uint64_t per[kSegs] = {};
+    const uint64_t stride = (uint64_t)blocks * 256;
+    for (unsigned b = 0; b < blocks; ++b) {
+        uint64_t items = 0;
+        for (uint64_t start = (uint64_t)b * 256; start < cap; start += stride)
+            items += std::min<uint64_t>(256, cap - start);
+        per[b % kSegs] += items;
+    }
+    uint64_t m = 0;
+    for (int g = 0; g < kSegs; ++g) m = std::max(m, per[g]);
+    return (unsigned)((m + 63) & ~uint64_t(63));
+}
+
+void Engine::initCtrl() {
+    std::memset(hCtrl, 0, sizeof(Ctrl));
+    hCtrl->capacity = capacity;
+    hCtrl->segCap = segCap;
+    hCtrl->connSegCap = 2 * segCap;
+    for (int k = 0; k < kMaxJobs; ++k) hCtrl->lastEmitIter[k] = -1;
+}
+
+gp_status Engine::resetJobAcc(int slot) {
+    launchFill64(dJobAcc + (size_t)slot * kSegs * kAccRow, (size_t)kSegs * kAccRow, 0ull, device().stream);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]) {
+    unsigned long long rows[kSegs * kAccRow];
+    GP_HIP(hipMemcpyAsync(rows, dJobAcc + (size_t)slot * kSegs * kAccRow, sizeof rows,
+                          hipMemcpyDeviceToHost, device().stream));
+    GP_HIP(hipStreamSynchronize(device().stream));
+    for (int k = 0; k < 5; ++k) ledger[k] = 0;
+    for (int k = 0; k < 4; ++k) counters[k] = 0;
+    for (int g = 0; g < kSegs; ++g) {
+        for (int k = 0; k < 5; ++k) ledger[k] += rows[g * kAccRow + k];
+        for (int k = 0; k < 4; ++k) counters[k] += rows[g * kAccRow + 5 + k];
+    }
+    return GP_OK;
+}
+
 gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
     if (!tunSet) { tun = defaultTuning(); tunSet = true; }
     DeviceState& d = device();
@@ -318,7 +364,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
     ls.blocksExtend = std::max(1, cus * tun.grid_waves_per_cu / 4);
     ls.blocksShade = std::max(1, cus * tun.grid_waves_per_cu / 4);
     const size_t threads = (size_t)ls.blocksExtend * 256;
-    const bool needQueues = (cap != capacity) || !dCtrl;
+    const unsigned sc = segmentBound(cap, (unsigned)ls.blocksShade);
+    const bool needQueues = (cap != capacity) || sc != segCap || !dCtrl;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     if (!needQueues && !needSpillAlloc && q.lds == lds && graphShape == ls.blocksExtend) return GP_OK;
     if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
@@ -331,18 +378,19 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
             return GP_OK;
         };
         gp_status st;
+        const size_t pcap = (size_t)kSegs * sc;
         for (int k = 0; k < 2; ++k) {
-            if ((st = alloc(cap * sizeof(float4), (void**)&q.ro[k])) != GP_OK) return st;
-            if ((st = alloc(cap * sizeof(float4), (void**)&q.rd[k])) != GP_OK) return st;
-            if ((st = alloc(cap * sizeof(float4), (void**)&q.lam[k])) != GP_OK) return st;
-            if ((st = alloc(cap * sizeof(float4), (void**)&q.beta[k])) != GP_OK) return st;
-            if ((st = alloc(cap * sizeof(uint4), (void**)&q.st[k])) != GP_OK) return st;
+            if ((st = alloc(pcap * sizeof(float4), (void**)&q.ro[k])) != GP_OK) return st;
+            if ((st = alloc(pcap * sizeof(float4), (void**)&q.rd[k])) != GP_OK) return st;
+            if ((st = alloc(pcap * sizeof(float4), (void**)&q.lam[k])) != GP_OK) return st;
+            if ((st = alloc(pcap * sizeof(float4), (void**)&q.beta[k])) != GP_OK) return st;
+            if ((st = alloc(pcap * sizeof(uint4), (void**)&q.st[k])) != GP_OK) return st;
         }
-        if ((st = alloc(cap * sizeof(float2), (void**)&q.hit)) != GP_OK) return st;
+        if ((st = alloc(pcap * sizeof(float2), (void**)&q.hit)) != GP_OK) return st;
         // Connections per iteration: one per emitted photon (direct view of the light)
-        // plus at most one per live path, and live paths include this iteration's
-        // emissions, so the connection queue holds twice the path capacity.
-        const size_t ccap = 2 * (size_t)cap;
+        // plus at most one per live path; k_emit and k_shade run the same grid, so a
+        // segment receives at most segCap of each.
+        const size_t ccap = 2 * pcap;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.co)) != GP_OK) return st;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.cd)) != GP_OK) return st;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.cx)) != GP_OK) return st;
@@ -350,13 +398,15 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
         GP_HIP(hipMalloc(&dJobs, sizeof(DevJob) * kMaxJobs));
         GP_HIP(hipHostMalloc(&hCtrl, sizeof(Ctrl)));
         GP_HIP(hipHostMalloc(&hJobs, sizeof(DevJob) * kMaxJobs));
-        GP_HIP(hipMalloc(&dJobEnergy, sizeof(unsigned long long) * 5 * kMaxJobs));
-        GP_HIP(hipMalloc(&dJobCounters, sizeof(unsigned long long) * 4 * kMaxJobs));
+        GP_HIP(hipMalloc(&dJobAcc, sizeof(unsigned long long) * kMaxJobs * kSegs * kAccRow));
         std::memset(hCtrl, 0, sizeof(Ctrl));
         std::memset(hJobs, 0, sizeof(DevJob) * kMaxJobs);
         capacity = cap;
+        segCap = sc;
         q.ctrl = dCtrl;
         q.jobs = dJobs;
+        q.segCap = sc;
+        q.connSegCap = 2 * sc;
     }
     if (needSpillAlloc) {
         const int depthAlloc = std::max(needSpill, spillDepth);
@@ -458,6 +508,7 @@ gp_status Engine::readCtrl() {
     hipStream_t s = device().stream;
     GP_HIP(hipMemcpyAsync(hCtrl, dCtrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
+    if (hCtrl->overflow) return fail(GP_ERR_CAPACITY, "wavefront queue segment overflow");
     if (tun.profile) return collectProfile();
     return GP_OK;
 }

@@ -83,6 +83,7 @@ struct Engine {
     gp_tuning tun{};
     bool tunSet = false;
     unsigned capacity = 0;
+    unsigned segCap = 0;                // path slots per queue segment
     int spillDepth = 0;
     size_t spillThreads = 0;
     Queues q{};
@@ -91,8 +92,10 @@ struct Engine {
     Ctrl* hCtrl = nullptr;              // pinned
     DevJob* dJobs = nullptr;
     DevJob* hJobs = nullptr;            // pinned
-    unsigned long long* dJobEnergy = nullptr;    // kMaxJobs * 5
-    unsigned long long* dJobCounters = nullptr;  // kMaxJobs * 4
+    unsigned long long* dJobAcc = nullptr;       // kMaxJobs * kSegs * kAccRow
+    gp_status resetJobAcc(int slot);             // zero a slot's rows (stream-ordered)
+    gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]);
+    void initCtrl();                            // zero the host control block, set capacities
     void* bufs[16] = {};
     int nbufs = 0;
     int parity = 0;
@@ -117,6 +120,7 @@ struct Engine {
 };
 Engine& engine();
 gp_tuning defaultTuning();
+unsigned segmentBound(unsigned cap, unsigned blocks);
 
 gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
                   const gp_trace_params& p, int slot, DevJob& J);

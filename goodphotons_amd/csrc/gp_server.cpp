@@ -67,9 +67,7 @@ gp_status uploadSlot(Engine& E, int slot) {
     GP_HIP(hipMemcpyAsync(&d->endPhoton[slot], &h->endPhoton[slot], sizeof(unsigned long long), hipMemcpyHostToDevice, s));
     GP_HIP(hipMemcpyAsync(&d->lastEmitIter[slot], &h->lastEmitIter[slot], sizeof(int32_t), hipMemcpyHostToDevice, s));
     GP_HIP(hipMemcpyAsync(&d->active[slot], &h->active[slot], sizeof(int32_t), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemsetAsync(E.dJobEnergy + 5 * (size_t)slot, 0, 5 * sizeof(unsigned long long), s));
-    GP_HIP(hipMemsetAsync(E.dJobCounters + 4 * (size_t)slot, 0, 4 * sizeof(unsigned long long), s));
-    return GP_OK;
+    return E.resetJobAcc(slot);
 }
 
 gp_status uploadOrder(Engine& E, gp_server* srv) {
@@ -143,8 +141,8 @@ gp_status retire(Engine& E, gp_server* srv, int32_t* finished) {
         const int last = h.lastEmitIter[slot];
         if (last >= 0 && (int64_t)h.iter >= (int64_t)last + J.desc.params.max_depth) {
             unsigned long long ev[5], cv[4];
-            GP_HIP(hipMemcpy(ev, E.dJobEnergy + 5 * (size_t)slot, sizeof ev, hipMemcpyDeviceToHost));
-            GP_HIP(hipMemcpy(cv, E.dJobCounters + 4 * (size_t)slot, sizeof cv, hipMemcpyDeviceToHost));
+            gp_status st = E.readJobAcc(slot, ev, cv);
+            if (st != GP_OK) return st;
             J.energy.emitted = (double)ev[0] / J.energyScale;
             J.energy.absorbed = (double)ev[1] / J.energyScale;
             J.energy.sensor = (double)ev[2] / J.energyScale;
@@ -226,13 +224,11 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     int maxDepthScene = 0;
     for (auto& kv : srv->jobs) maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
     const int lds = E.tun.lds_stack;
-    const bool fresh = !srv->engineReady || E.capacity != (unsigned)E.tun.queue_capacity;
+    const unsigned oldCap = E.capacity, oldSeg = E.segCap;
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds))) != GP_OK) return st;
+    const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg;
     if (fresh) {
-        std::memset(E.hCtrl, 0, sizeof(Ctrl));
-        E.hCtrl->capacity = E.capacity;
-        E.hCtrl->connCapacity = 2 * E.capacity;
-        for (int k = 0; k < kMaxJobs; ++k) E.hCtrl->lastEmitIter[k] = -1;
+        E.initCtrl();
         GP_HIP(hipMemcpy(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice));
         E.parity = 0;
         srv->engineReady = true;

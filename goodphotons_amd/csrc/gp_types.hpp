@@ -3,7 +3,8 @@
 //   BVH        child-pair nodes, 64 B each (both children's boxes + links)
 //   triangles  3 x float4 per triangle in leaf order (vertices; .w = prim id,
 //              material id), plus 1 x float4 geometric normal (.w = material)
-//   paths      structure-of-arrays queues of float4/uint4 (80 B per path)
+//   paths      structure-of-arrays queues of float4/uint4 (80 B per path), split
+//              into kSegs segments, each with its own append counter
 //   hits       one float2 per path (t, leaf-order triangle index)
 //   camera     connections: 3 x float4 per record (48 B)
 //   films      int64 fixed-point (X, Y, Z, hits) per pixel, 32 B
@@ -19,6 +20,15 @@ constexpr int kMaxJobs = 256;         // job slots in one wavefront
 constexpr int kLambdaBins = 470;      // 1-nm bins over [360, 830)
 constexpr float kRayEps = 1e-6f;      // origin offset and tmin, as the reference CPU path
 constexpr int kLeafMax = 4;           // triangles per BVH leaf
+// Queues are split into kSegs segments, each appended to through its own counter
+// (its own 128-byte line): workgroup b appends to segment b % kSegs. Same-address
+// atomics serialise (~12 ns each on MI355X), so eight counters instead of one cut
+// the append cost of a launch eightfold; and because a segment only receives the
+// items of every eighth workgroup, its fill is bounded without any assumption about
+// where the hardware places workgroups.
+constexpr int kSegs = 8;
+constexpr int kLine = 32;             // u32 per 128-byte line
+constexpr int kAccRow = 16;           // u64 per per-segment ledger/counter row
 
 // Leaf links are negative: ~((first << 3) | (count - 1)).
 inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (count - 1)); }
@@ -66,9 +76,11 @@ struct DevJob {
     long long* film;         // res*4 int64
     double filmScale;        // contribution -> integer units
     unsigned long long filmQmax;   // saturation value of one splat
-    long long* energy;       // 5 int64 (emitted, absorbed, sensor, escaped, residual)
+    // kSegs rows of kAccRow u64: [0..4] energy (emitted, absorbed, sensor, escaped,
+    // residual, fixed point), [5..8] counters (photons, segments, connections,
+    // splats); workgroup b adds into row b % kSegs
+    unsigned long long* acc;
     double energyScale;
-    unsigned long long* counters;  // photons, segments, connections, splats
     // path settings
     unsigned long long seedBase, salt;
     int32_t heroC, maxDepth, rrDepth, pad0;
@@ -76,15 +88,19 @@ struct DevJob {
 
 // Device-side wavefront control block.
 struct Ctrl {
-    unsigned int nPaths[2];     // paths in queue 0 / 1
-    unsigned int nConn;         // camera connections queued this iteration
+    unsigned int pathCount[2][kSegs][kLine];   // per segment of path queue 0 / 1 (one line each)
+    unsigned int connCount[kSegs][kLine];      // per segment of the connection queue
     unsigned int nEmit;         // photons emitted this iteration
-    unsigned int emitBase;      // queue slot of the first emitted photon
     unsigned int iter;          // iterations run
-    int32_t nJobs;              // unused slots above this are never read
-    unsigned int capacity;      // path queue capacity
-    unsigned int connCapacity;
-    unsigned int pad[3];
+    int32_t nJobs;              // job slots in use are below this
+    unsigned int capacity;      // path queue capacity (total over segments)
+    unsigned int segCap;        // path slots per segment
+    unsigned int connSegCap;    // connection slots per segment
+    unsigned int overflow;      // set by a kernel that found a segment full (never expected)
+    unsigned int pad;
+    unsigned int segBase[kSegs];        // survivors already in each segment (emission goes after)
+    unsigned int segEmit[kSegs + 1];    // prefix of this iteration's emission over segments
+    unsigned int pad2[7];
     // per job
     unsigned long long nextPhoton[kMaxJobs];   // next photon index to emit (job-relative)
     unsigned long long endPhoton[kMaxJobs];    // one past the last photon
@@ -92,7 +108,7 @@ struct Ctrl {
     int32_t active[kMaxJobs];                  // 1 = emitting / in flight
     int32_t order[kMaxJobs];                   // job slots in emission priority order
     int32_t nOrder;                            // entries of `order` in use
-    int32_t pad2[3];
+    int32_t pad3[3];
     // this iteration's emission, indexed by position k in `order`
     unsigned int emitPrefix[kMaxJobs + 1];     // prefix of photons emitted before order[k]
     int32_t emitJob[kMaxJobs];                 // = order[k]
