@@ -565,36 +565,74 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ connect
-// Splat cache: a per-workgroup open-addressing table in LDS keyed by (job, pixel).
+// Splat table: a per-workgroup open-addressing table in LDS keyed by (job, pixel).
 // Unoccluded connections add their integer X, Y, Z and hit count into it with LDS
-// 64-bit atomics; the table is flushed to the film once per workgroup. Pixels that
-// many connections hit (the emitter's direct view, caustic foci) then cost one
-// global atomic per workgroup instead of one per connection. Integer sums are
-// order-independent, so the film is bit-identical to splatting directly.
+// 64-bit atomics. When a connection finds no slot within kSplatProbe probes, the
+// workgroup flushes the whole table to the film and starts a fresh one, so every
+// film update goes through the table. Two properties of the memory system shape
+// this:
+//   - film atomics are limited by requests, not operations: four lanes adding the
+//     four words of one pixel in one instruction cost one request, four lanes each
+//     adding to their own pixel cost four (measured ~91 vs ~23 G atomics/s on
+//     MI355X), so the flush hands (slot, word) pairs to consecutive lanes;
+//   - atomics on one address serialise (~12 ns each), and pixels that many
+//     connections hit (the emitter's direct view, caustic foci) would queue on a few
+//     addresses; the table sums them first.
+// Integer sums are order-independent, so the film is bit-identical to splatting
+// each connection directly.
 constexpr int kSplatSlots = 256;
-constexpr unsigned kSplatEmpty = 0xffffffffu;
-constexpr int kSplatPixelBits = 22;   // pixels < 4M and jobs < 1024 use the cache
+constexpr int kSplatProbe = 8;
+constexpr unsigned long long kSplatEmpty = ~0ull;
 
-__device__ __forceinline__ void splatGlobal(long long* film, int pix, unsigned long long X,
-                                            unsigned long long Y, unsigned long long Z,
-                                            unsigned long long H) {
-    unsigned long long* f = (unsigned long long*)(film + 4 * (size_t)pix);
-    if (X) atomicAdd(f + 0, X);
-    if (Y) atomicAdd(f + 1, Y);
-    if (Z) atomicAdd(f + 2, Z);
-    atomicAdd(f + 3, H);
+// Slot of `key` (claimed if absent) within maxProbe linear probes, or -1.
+__device__ __forceinline__ int splatSlot(unsigned long long* sKey, unsigned long long key, int maxProbe) {
+    const unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 56);
+    for (int p = 0; p < maxProbe; ++p) {
+        const int s = (int)((h + (unsigned)p) & (kSplatSlots - 1));
+        unsigned long long cur = __hip_atomic_load(&sKey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == kSplatEmpty) cur = atomicCAS(&sKey[s], kSplatEmpty, key);
+        if (cur == kSplatEmpty || cur == key) return s;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void splatAdd(unsigned long long* sVal, int s, unsigned long long X,
+                                         unsigned long long Y, unsigned long long Z, unsigned long long H) {
+    if (X) atomicAdd(&sVal[4 * s + 0], X);
+    if (Y) atomicAdd(&sVal[4 * s + 1], Y);
+    if (Z) atomicAdd(&sVal[4 * s + 2], Z);
+    atomicAdd(&sVal[4 * s + 3], H);
+}
+
+// Called by every thread of the workgroup after a barrier: adds every occupied
+// slot into its film, lane t taking word t % 4 of slot t / 4 (+ 64 per pass).
+__device__ __forceinline__ void flushSplats(const DevJob* jobs, const unsigned long long* sKey,
+                                            const unsigned long long* sVal) {
+    const int w = threadIdx.x & 3;
+    for (int s = threadIdx.x >> 2; s < kSplatSlots; s += blockDim.x >> 2) {
+        const unsigned long long key = sKey[s];
+        if (key == kSplatEmpty) continue;
+        const unsigned long long v = sVal[4 * s + w];
+        if (v) {
+            unsigned long long* f = (unsigned long long*)jobs[key >> 32].film;
+            atomicAdd(f + 4 * (size_t)(key & 0xffffffffull) + w, v);
+        }
+    }
+}
+
+__device__ __forceinline__ void clearSplats(unsigned long long* sKey, unsigned long long* sVal) {
+    for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
+        sKey[k] = kSplatEmpty;
+        sVal[4 * k + 0] = 0; sVal[4 * k + 1] = 0; sVal[4 * k + 2] = 0; sVal[4 * k + 3] = 0;
+    }
 }
 
 template <int SD>
 __global__ void __launch_bounds__(256) k_connect(Queues q) {
     __shared__ int sIdx[SD * 256];
-    __shared__ unsigned sKey[kSplatSlots];
+    __shared__ unsigned long long sKey[kSplatSlots];
     __shared__ unsigned long long sVal[kSplatSlots * 4];
-    for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
-        sKey[k] = kSplatEmpty;
-        sVal[4 * k + 0] = 0; sVal[4 * k + 1] = 0; sVal[4 * k + 2] = 0; sVal[4 * k + 3] = 0;
-    }
-    __syncthreads();
+    clearSplats(sKey, sVal);
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->connCount, q.connSegCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
@@ -603,56 +641,50 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
                          q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
     __shared__ unsigned long long cTab[kMaxJobs * 2];
-    initJobTable<2>(cTab, nJobs);
+    initJobTable<2>(cTab, nJobs);   // (its barrier also orders clearSplats)
     JobAcc<2> cacc;   // connections, splats
     AccAddr ca{q.jobs, kAccPhotons + 2};
-    for (unsigned li = gtid; li < n; li += stride) {
-        const unsigned i = sm.slot(li);
-        const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
-        const unsigned tag = (unsigned)__float_as_int(cx.w);
-        const int j = (int)(tag & 0xffffu);
-        const DevJob& J = q.jobs[j];
-        accumulate(cacc, j, cTab);
-        cacc.v[0] += 1;
-        RayPre r = prepareRay(xyz(co), xyz(cd));
-        const bool occ = traceAny<SD, false>(J.nodes, J.tris, r, kRayEps, co.w, stk);
-        if (!occ) {
-            cacc.v[1] += 1;
-            const int pix = __float_as_int(cd.w);
-            const unsigned long long X = min(toFixed((double)cx.x, J.filmScale), J.filmQmax);
-            const unsigned long long Y = min(toFixed((double)cx.y, J.filmScale), J.filmQmax);
-            const unsigned long long Z = min(toFixed((double)cx.z, J.filmScale), J.filmQmax);
-            const unsigned long long H = (unsigned long long)(tag >> 16);
-            int slot = -1;
-            if (pix < (1 << kSplatPixelBits) && j < (1 << (32 - kSplatPixelBits))) {
-                const unsigned key = ((unsigned)j << kSplatPixelBits) | (unsigned)pix;
-                const unsigned h = (key * 2654435761u) >> 24;
-                for (int probe = 0; probe < 4; ++probe) {
-                    const int s = (int)((h + probe) & (kSplatSlots - 1));
-                    unsigned cur = sKey[s];
-                    if (cur == kSplatEmpty) cur = atomicCAS(&sKey[s], kSplatEmpty, key);
-                    if (cur == kSplatEmpty || cur == key) { slot = s; break; }
-                }
-            }
-            if (slot >= 0) {
-                if (X) atomicAdd(&sVal[4 * slot + 0], X);
-                if (Y) atomicAdd(&sVal[4 * slot + 1], Y);
-                if (Z) atomicAdd(&sVal[4 * slot + 2], Z);
-                atomicAdd(&sVal[4 * slot + 3], H);
-            } else {
-                splatGlobal(J.film, pix, X, Y, Z, H);
+    // Rounds are workgroup-uniform so the table can be flushed between them.
+    for (unsigned base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const unsigned li = base + threadIdx.x;
+        bool pending = false;
+        unsigned long long key = 0, X = 0, Y = 0, Z = 0, H = 0;
+        if (li < n) {
+            const unsigned i = sm.slot(li);
+            const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
+            const unsigned tag = (unsigned)__float_as_int(cx.w);
+            const int j = (int)(tag & 0xffffu);
+            const DevJob& J = q.jobs[j];
+            accumulate(cacc, j, cTab);
+            cacc.v[0] += 1;
+            RayPre r = prepareRay(xyz(co), xyz(cd));
+            const bool occ = traceAny<SD, false>(J.nodes, J.tris, r, kRayEps, co.w, stk);
+            if (!occ) {
+                cacc.v[1] += 1;
+                const unsigned pix = (unsigned)__float_as_int(cd.w);
+                X = min(toFixed((double)cx.x, J.filmScale), J.filmQmax);
+                Y = min(toFixed((double)cx.y, J.filmScale), J.filmQmax);
+                Z = min(toFixed((double)cx.z, J.filmScale), J.filmQmax);
+                H = (unsigned long long)(tag >> 16);
+                key = ((unsigned long long)j << 32) | pix;
+                const int s = splatSlot(sKey, key, kSplatProbe);
+                if (s >= 0) splatAdd(sVal, s, X, Y, Z, H);
+                else pending = true;
             }
         }
+        if (__syncthreads_or(pending)) {
+            flushSplats(q.jobs, sKey, sVal);
+            __syncthreads();
+            clearSplats(sKey, sVal);
+            __syncthreads();
+            // At most one key per thread goes into the empty table, so a full scan
+            // always finds its key or a free slot.
+            if (pending) splatAdd(sVal, splatSlot(sKey, key, kSplatSlots), X, Y, Z, H);
+        }
     }
-    flushJobTable(cacc, cTab, nJobs, ca);
+    flushJobTable(cacc, cTab, nJobs, ca);   // ends with a barrier-ordered read of cTab
     __syncthreads();
-    for (int k = threadIdx.x; k < kSplatSlots; k += blockDim.x) {
-        const unsigned key = sKey[k];
-        if (key == kSplatEmpty) continue;
-        const int j = (int)(key >> kSplatPixelBits);
-        const int pix = (int)(key & ((1u << kSplatPixelBits) - 1));
-        splatGlobal(q.jobs[j].film, pix, sVal[4 * k + 0], sVal[4 * k + 1], sVal[4 * k + 2], sVal[4 * k + 3]);
-    }
+    flushSplats(q.jobs, sKey, sVal);
 }
 
 // ------------------------------------------------------------------ probes

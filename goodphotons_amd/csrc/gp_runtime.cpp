@@ -259,10 +259,10 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
 // ------------------------------------------------------------------ engine
 gp_tuning defaultTuning() {
     gp_tuning t;
-    t.queue_capacity = 1 << 20;
+    t.queue_capacity = 1 << 22;
     t.block_extend = 256;
     t.block_shade = 256;
-    t.grid_waves_per_cu = 32;
+    t.grid_waves_per_cu = 16;
     t.iters_per_graph = 8;
     t.use_graph = 1;
     t.lds_stack = 16;
