@@ -52,6 +52,23 @@ def kernel_bytes(st):
     }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_traffic.json")
+
+
+def measured_traffic(kernel, tuning, photons):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes of this workload (scaled by the copy calibration), when they
+    were taken at the same tuning and photon count; otherwise None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if photons != 1 << 26 or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
+        return None
+    return rec["kernels"][kernel]["hbm_bytes_per_dispatch"]
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -184,8 +201,10 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     total_bytes = sum(kb.values())
     total_ms = sum(kms.values())
+    traffic = measured_traffic(dom, {k: getattr(tun, k) for k, _ in tun._fields_}, N)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                "traffic_source": "profiles/c1_traffic.json" if traffic is not None else None,
                 "avg_launch_ms": avg_ms, "launches": kl[dom],
                 "algorithmic_bytes_per_launch": bytes_per_launch,
                 "kernel_ms": kms, "pipeline_bytes_per_segment": total_bytes / max(1, pst["segments"]),
