@@ -34,6 +34,16 @@ __device__ __forceinline__ F3 reflectDir(F3 d, F3 n) { return d - n * (2.0f * do
 __device__ __forceinline__ F3 load3(const float* p) { return F3{p[0], p[1], p[2]}; }
 __device__ __forceinline__ F3 xyz(float4 v) { return F3{v.x, v.y, v.z}; }
 
+// Loads through a global-address-space pointer. Scene tables are reached through
+// pointers stored in the job record, which the compiler cannot place in an address
+// space; a plain dereference then becomes a FLAT load (address aperture checks, and
+// it shares the LDS wait counter with the traversal stack). These force GLOBAL loads.
+typedef float gp_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+    const gp_f4v v = *(const __attribute__((address_space(1))) gp_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Orthonormal basis around unit n (Duff et al. 2017, the reference's onb).
 __device__ __forceinline__ void frame(F3 n, F3& t, F3& b) {
     float sign = copysignf(1.0f, n.z);
@@ -216,6 +226,11 @@ __device__ __forceinline__ bool hitTri(const RayPre& r, F3 v0, F3 v1, F3 v2, flo
     float det = U + V + W;
     if (det == 0.0f) return false;
     float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    // Cheap rejections ahead of the division, each implying the exact test below
+    // fails: t = T / det <= 0 when the signs differ (tmin > 0), and t > tmax when
+    // |T| exceeds tmax |det| by more than the ~2^-22 relative rounding of the two.
+    if (tmin > 0.0f && !((T > 0.0f) == (det > 0.0f) && T != 0.0f)) return false;
+    if (fabsf(T) > tmax * fabsf(det) * 1.0000010f) return false;
     float t = T * (1.0f / det);
     if (!(t >= tmin && t < tmax)) return false;
     tOut = t;
@@ -250,21 +265,26 @@ __device__ __forceinline__ PairHit slabPair(const RayPre& r, float4 a, float4 b,
     return p;
 }
 
-// Traversal stack: the top `SD` entries live in LDS at [slot * stride + lane], deeper
+// Traversal kernels run workgroups of this many threads; the LDS stack is laid out
+// with this stride so lane addresses need no multiply.
+constexpr int kTravThreads = 256;
+
+// Traversal stack: the top `SD` entries live in LDS at [slot * 256 + lane], deeper
 // entries spill to a per-thread global area. The host guarantees SD + spill depth
-// >= the tree's depth, and each level pushes at most one entry, so it never overflows.
-// WITH_T = false keeps node indices only (any-hit needs no entry distances).
-template <int SD, bool WITH_T = true>
+// >= the tree's depth, and each level pushes at most one entry, so it never
+// overflows. With WITH_T an entry is (node, entry distance) in one 64-bit LDS word;
+// without (any-hit) just the node. SPILL = false is the variant for trees that fit
+// in LDS (no spill area allocated, no spill branches).
+template <int SD, bool WITH_T = true, bool SPILL = true>
 struct Stack {
-    int* sidx;            // LDS, SD * stride ints
-    float* st;            // LDS, SD * stride floats (unused without WITH_T)
-    int stride;
-    int2* spill;          // global, spillDepth entries for this thread (may be null)
+    int2* e2;             // LDS, this lane's column of SD entries (WITH_T)
+    int* e1;              // LDS, this lane's column of SD entries (!WITH_T)
+    int2* spill;          // global, spillDepth entries for this thread (SPILL)
     int sp;
     __device__ __forceinline__ void push(int node, float t) {
-        if (sp < SD) {
-            sidx[sp * stride] = node;
-            if (WITH_T) st[sp * stride] = t;
+        if (!SPILL || sp < SD) {
+            if (WITH_T) e2[sp * kTravThreads] = make_int2(node, __float_as_int(t));
+            else e1[sp * kTravThreads] = node;
         } else {
             spill[sp - SD] = make_int2(node, __float_as_int(t));
         }
@@ -272,29 +292,59 @@ struct Stack {
     }
     __device__ __forceinline__ void pop(int& node, float& t) {
         --sp;
-        if (sp < SD) {
-            node = sidx[sp * stride];
-            t = WITH_T ? st[sp * stride] : 0.0f;
+        if (!SPILL || sp < SD) {
+            if (WITH_T) {
+                const int2 v = e2[sp * kTravThreads];
+                node = v.x;
+                t = __int_as_float(v.y);
+            } else {
+                node = e1[sp * kTravThreads];
+                t = 0.0f;
+            }
         } else {
-            int2 v = spill[sp - SD];
-            node = v.x;
-            t = __int_as_float(v.y);
+            // a distinct load form keeps the compiler from merging the LDS and the
+            // global load into one FLAT load through a selected pointer
+            const unsigned long long v =
+                __builtin_nontemporal_load((const unsigned long long*)(spill + (sp - SD)));
+            node = (int)(unsigned)(v & 0xffffffffull);
+            t = __int_as_float((int)(unsigned)(v >> 32));
         }
     }
 };
 
+// Triangles of one leaf: all loads are issued before the first test, so a leaf of k
+// triangles waits for memory once, not k times.
+struct LeafTris {
+    float4 v[kLeafMax][3];
+};
+__device__ __forceinline__ void loadLeaf(const float4* __restrict__ tris, int first, int count, LeafTris& L) {
+#pragma unroll
+    for (int k = 0; k < kLeafMax; ++k) {
+        if (k < count) {
+            const float4* tp = tris + 3 * (first + k);
+            L.v[k][0] = ldg4(tp);
+            L.v[k][1] = ldg4(tp + 1);
+            L.v[k][2] = ldg4(tp + 2);
+        }
+    }
+}
+
 // Closest hit. Returns the leaf-order triangle index (or -1) and its distance.
-template <int SD>
+// Front-to-back: at a node with both children hit, the nearer is visited and the
+// farther pushed with its entry distance; a popped entry farther than the current
+// hit is skipped. Triangles of a leaf are tested in order; a later triangle only
+// replaces the hit if strictly closer.
+template <int SD, bool SPILL>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
-                                            float tmin, float& tmax, Stack<SD>& stk) {
+                                            float tmin, float& tmax, Stack<SD, true, SPILL>& stk) {
     int best = -1;
     int node = 0;
     stk.sp = 0;
     while (true) {
         if (node >= 0) {
             const float4* nd = nodes + 4 * node;
-            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+            const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
             int left = __float_as_int(a.w), right = __float_as_int(b.w);
             if (ph.hl && ph.hr) {
@@ -306,13 +356,14 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
             if (ph.hl) { node = left; continue; }
             if (ph.hr) { node = right; continue; }
         } else {
-            int code = ~node;
-            int first = code >> 3, count = (code & 7) + 1;
-            for (int k = 0; k < count; ++k) {
-                const float4* tp = tris + 3 * (first + k);
-                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+            const int code = ~node;
+            const int first = code >> 3, count = (code & 7) + 1;
+            LeafTris L;
+            loadLeaf(tris, first, count, L);
+#pragma unroll
+            for (int k = 0; k < kLeafMax; ++k) {
                 float t;
-                if (hitTri(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, t)) {
+                if (k < count && hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t)) {
                     tmax = t;
                     best = first + k;
                 }
@@ -331,16 +382,16 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
 }
 
 // Any hit on [tmin, tmax).
-template <int SD, bool WT>
+template <int SD, bool SPILL>
 __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                                          const float4* __restrict__ tris, const RayPre& r,
-                                         float tmin, float tmax, Stack<SD, WT>& stk) {
+                                         float tmin, float tmax, Stack<SD, false, SPILL>& stk) {
     int node = 0;
     stk.sp = 0;
     while (true) {
         if (node >= 0) {
             const float4* nd = nodes + 4 * node;
-            float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+            const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
             int left = __float_as_int(a.w), right = __float_as_int(b.w);
             if (ph.hl && ph.hr) {
@@ -352,14 +403,17 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
             if (ph.hl) { node = left; continue; }
             if (ph.hr) { node = right; continue; }
         } else {
-            int code = ~node;
-            int first = code >> 3, count = (code & 7) + 1;
-            for (int k = 0; k < count; ++k) {
-                const float4* tp = tris + 3 * (first + k);
-                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+            const int code = ~node;
+            const int first = code >> 3, count = (code & 7) + 1;
+            LeafTris L;
+            loadLeaf(tris, first, count, L);
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < kLeafMax; ++k) {
                 float t;
-                if (hitTri(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, t)) return true;
+                if (k < count && !hit) hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
             }
+            if (hit) return true;
         }
         if (stk.sp == 0) break;
         float t;

@@ -357,16 +357,15 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ extend
-template <int SD>
-__global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
-    __shared__ int sIdx[SD * 256];
-    __shared__ float sT[SD * 256];
+template <int SD, bool SPILL>
+__global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
+    __shared__ int2 sStk[SD * kTravThreads];
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
-                  q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    Stack<SD, true, SPILL> stk{sStk + threadIdx.x, nullptr,
+                               SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
     __shared__ unsigned long long sTab[kMaxJobs];
     initJobTable<1>(sTab, nJobs);
@@ -384,7 +383,7 @@ __global__ void __launch_bounds__(256) k_extend(Queues q, int cur) {
             const float4 d = q.rd[cur][i];
             RayPre r = prepareRay(xyz(o), xyz(d));
             float tmax = INFINITY;
-            best = traceClosest<SD>(J.nodes, J.tris, r, kRayEps, tmax, stk);
+            best = traceClosest<SD, SPILL>(J.nodes, J.tris, r, kRayEps, tmax, stk);
             t = tmax;
             accumulate(sacc, j, sTab);
             sacc.v[0] += 1;
@@ -627,9 +626,9 @@ __device__ __forceinline__ void clearSplats(unsigned long long* sKey, unsigned l
     }
 }
 
-template <int SD>
-__global__ void __launch_bounds__(256) k_connect(Queues q) {
-    __shared__ int sIdx[SD * 256];
+template <int SD, bool SPILL>
+__global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
+    __shared__ int sStk[SD * kTravThreads];
     __shared__ unsigned long long sKey[kSplatSlots];
     __shared__ unsigned long long sVal[kSplatSlots * 4];
     clearSplats(sKey, sVal);
@@ -637,8 +636,8 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
     const unsigned n = loadSegMap(q.ctrl->connCount, q.connSegCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD, false> stk{sIdx + threadIdx.x, nullptr, (int)blockDim.x,
-                         q.spill ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    Stack<SD, false, SPILL> stk{nullptr, sStk + threadIdx.x,
+                                SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
     __shared__ unsigned long long cTab[kMaxJobs * 2];
     initJobTable<2>(cTab, nJobs);   // (its barrier also orders clearSplats)
@@ -658,7 +657,7 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
             accumulate(cacc, j, cTab);
             cacc.v[0] += 1;
             RayPre r = prepareRay(xyz(co), xyz(cd));
-            const bool occ = traceAny<SD, false>(J.nodes, J.tris, r, kRayEps, co.w, stk);
+            const bool occ = traceAny<SD, SPILL>(J.nodes, J.tris, r, kRayEps, co.w, stk);
             if (!occ) {
                 cacc.v[1] += 1;
                 const unsigned pix = (unsigned)__float_as_int(cd.w);
@@ -688,42 +687,40 @@ __global__ void __launch_bounds__(256) k_connect(Queues q) {
 }
 
 // ------------------------------------------------------------------ probes
-template <int SD>
-__global__ void __launch_bounds__(256) k_probe_closest(const float4* nodes, const float4* tris,
-                                                      const float* rays, long long n, float tmin,
-                                                      int* prim, float* tout, int2* spill,
-                                                      int spillDepth) {
-    __shared__ int sIdx[SD * 256];
-    __shared__ float sT[SD * 256];
+template <int SD, bool SPILL>
+__global__ void __launch_bounds__(kTravThreads) k_probe_closest(const float4* nodes, const float4* tris,
+                                                               const float* rays, long long n, float tmin,
+                                                               int* prim, float* tout, int2* spill,
+                                                               int spillDepth) {
+    __shared__ int2 sStk[SD * kTravThreads];
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
-                  spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
+    Stack<SD, true, SPILL> stk{sStk + threadIdx.x, nullptr,
+                               SPILL ? spill + (size_t)gtid * spillDepth : nullptr, 0};
     for (long long i = gtid; i < n; i += stride) {
         const float* rp = rays + 6 * i;
         RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
         float tmax = INFINITY;
-        int best = traceClosest<SD>(nodes, tris, r, tmin, tmax, stk);
+        int best = traceClosest<SD, SPILL>(nodes, tris, r, tmin, tmax, stk);
         prim[i] = best >= 0 ? __float_as_int(tris[3 * best].w) : -1;
         tout[i] = best >= 0 ? tmax : INFINITY;
     }
 }
 
-template <int SD>
-__global__ void __launch_bounds__(256) k_probe_any(const float4* nodes, const float4* tris,
-                                                  const float* rays, const float* tmax, long long n,
-                                                  float tmin, unsigned char* occ, int2* spill,
-                                                  int spillDepth) {
-    __shared__ int sIdx[SD * 256];
-    __shared__ float sT[SD * 256];
+template <int SD, bool SPILL>
+__global__ void __launch_bounds__(kTravThreads) k_probe_any(const float4* nodes, const float4* tris,
+                                                           const float* rays, const float* tmax, long long n,
+                                                           float tmin, unsigned char* occ, int2* spill,
+                                                           int spillDepth) {
+    __shared__ int sStk[SD * kTravThreads];
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    Stack<SD, false> stk{sIdx + threadIdx.x, sT + threadIdx.x, (int)blockDim.x,
-                         spill ? spill + (size_t)gtid * spillDepth : nullptr, 0};
+    Stack<SD, false, SPILL> stk{nullptr, sStk + threadIdx.x,
+                                SPILL ? spill + (size_t)gtid * spillDepth : nullptr, 0};
     for (long long i = gtid; i < n; i += stride) {
         const float* rp = rays + 6 * i;
         RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
-        occ[i] = traceAny<SD, false>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
+        occ[i] = traceAny<SD, SPILL>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
     }
 }
 
@@ -742,44 +739,53 @@ void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) 
 void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
 }
+// Kernel instances by LDS stack depth and spill use.
+#define GP_BY_STACK(LDS, SPILL, CALL) \
+    do {                                   \
+        if (SPILL) {                       \
+            switch (LDS) {                 \
+                case 8: CALL(8, true); break;   \
+                case 24: CALL(24, true); break; \
+                case 32: CALL(32, true); break; \
+                default: CALL(16, true); break; \
+            }                              \
+        } else {                           \
+            switch (LDS) {                 \
+                case 8: CALL(8, false); break;   \
+                case 24: CALL(24, false); break; \
+                case 32: CALL(32, false); break; \
+                default: CALL(16, false); break; \
+            }                              \
+        }                                  \
+    } while (0)
+
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-    switch (q.lds) {
-        case 8: hipLaunchKernelGGL(k_extend<8>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
-        case 24: hipLaunchKernelGGL(k_extend<24>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
-        case 32: hipLaunchKernelGGL(k_extend<32>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
-        default: hipLaunchKernelGGL(k_extend<16>, dim3(ls.blocksExtend), dim3(256), 0, s, q, cur); break;
-    }
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
+    GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+#undef GP_CALL
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
-    switch (q.lds) {
-        case 8: hipLaunchKernelGGL(k_connect<8>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
-        case 24: hipLaunchKernelGGL(k_connect<24>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
-        case 32: hipLaunchKernelGGL(k_connect<32>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
-        default: hipLaunchKernelGGL(k_connect<16>, dim3(ls.blocksExtend), dim3(256), 0, s, q); break;
-    }
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
+    GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+#undef GP_CALL
 }
 void launchProbeClosest(const float4* nodes, const float4* tris, const float* rays, long long n,
                         float tmin, int* prim, float* t, int2* spill, int spillDepth, int lds,
                         int blocks, int threads, hipStream_t s) {
     (void)threads;
-    switch (lds) {
-        case 8: hipLaunchKernelGGL(k_probe_closest<8>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
-        case 24: hipLaunchKernelGGL(k_probe_closest<24>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
-        case 32: hipLaunchKernelGGL(k_probe_closest<32>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
-        default: hipLaunchKernelGGL(k_probe_closest<16>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth); break;
-    }
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_probe_closest<SD, SP>), dim3(blocks), dim3(kTravThreads), 0, s, nodes, tris, rays, n, tmin, prim, t, spill, spillDepth)
+    GP_BY_STACK(lds, spillDepth > 0, GP_CALL);
+#undef GP_CALL
 }
 void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, const float* tmax,
                     long long n, float tmin, unsigned char* occ, int2* spill, int spillDepth,
                     int lds, int blocks, int threads, hipStream_t s) {
     (void)threads;
-    switch (lds) {
-        case 8: hipLaunchKernelGGL(k_probe_any<8>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
-        case 24: hipLaunchKernelGGL(k_probe_any<24>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
-        case 32: hipLaunchKernelGGL(k_probe_any<32>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
-        default: hipLaunchKernelGGL(k_probe_any<16>, dim3(blocks), dim3(256), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth); break;
-    }
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_probe_any<SD, SP>), dim3(blocks), dim3(kTravThreads), 0, s, nodes, tris, rays, tmax, n, tmin, occ, spill, spillDepth)
+    GP_BY_STACK(lds, spillDepth > 0, GP_CALL);
+#undef GP_CALL
 }
+#undef GP_BY_STACK
 void launchFill64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s) {
     if (n == 0) return;
     size_t blocks = (n + 255) / 256;

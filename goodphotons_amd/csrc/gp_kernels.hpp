@@ -25,6 +25,7 @@ struct Queues {
     int2* spill;     // traversal stack spill, spillDepth entries per thread
     int spillDepth;
     int lds;         // LDS stack entries per lane (8, 16, 24 or 32)
+    int useSpill;    // 1: the traced scenes are deeper than the LDS stack (spill variant)
     unsigned segCap, connSegCap;
 };
 

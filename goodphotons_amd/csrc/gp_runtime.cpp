@@ -367,7 +367,13 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
     const unsigned sc = segmentBound(cap, (unsigned)ls.blocksShade);
     const bool needQueues = (cap != capacity) || sc != segCap || !dCtrl;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
-    if (!needQueues && !needSpillAlloc && q.lds == lds && graphShape == ls.blocksExtend) return GP_OK;
+    // The kernels without spill code keep the whole stack in LDS: they may only run
+    // when every traced scene fits (needSpill == 0). A change of variant re-captures
+    // the graph.
+    const int useSpill = needSpill > 0 ? 1 : 0;
+    if (!needQueues && !needSpillAlloc && q.lds == lds && q.useSpill == useSpill &&
+        graphShape == ls.blocksExtend)
+        return GP_OK;
     if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
     if (needQueues) {
         releaseQueues();
@@ -419,6 +425,7 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
     }
     q.spillDepth = spillDepth;
     q.lds = lds;
+    q.useSpill = useSpill;
     graphShape = ls.blocksExtend;
     return GP_OK;
 }
