@@ -52,6 +52,9 @@ class Library:
         L.gp_film_download_raw.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.gp_film_destroy.argtypes = [vp]
         L.gp_film_destroy.restype = None
+        L.gp_film_to_linear.argtypes = [vp, C.c_int64, C.POINTER(C.c_float)]
+        L.gp_film_tonemap.argtypes = [vp, C.c_int64, C.c_double, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_uint8), C.POINTER(C.c_double)]
         L.gp_trace.argtypes = [vp, C.POINTER(abi.CameraDesc), vp, C.POINTER(abi.TraceParams),
                                C.POINTER(abi.Energy), C.POINTER(abi.TraceStats)]
         L.gp_probe_closest_hit.argtypes = [vp, C.POINTER(C.c_float), C.c_int64, C.c_float,
@@ -67,6 +70,7 @@ class Library:
         L.gp_server_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         L.gp_server_take_result.argtypes = [vp, C.c_int64, C.POINTER(abi.JobResult),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gp_server_poll.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]
         L.gp_server_pending.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.gp_server_destroy.argtypes = [vp]
         L.gp_server_destroy.restype = None
@@ -177,6 +181,24 @@ class Film:
     def reset(self):
         self.L.check(self.L.lib.gp_film_reset(self.h))
 
+    def to_linear(self, n_photons):
+        """Scene-linear sRGB, float32 (res_y, res_x, 3), row 0 = bottom (PFM order)."""
+        out = np.zeros((self.ry, self.rx, 3), np.float32)
+        self.L.check(self.L.lib.gp_film_to_linear(self.h, int(n_photons),
+                                                  out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def tonemap(self, n_photons, exposure_comp=0.0, absolute=False, anchor=None):
+        """8-bit sRGB (res_y, res_x, 3), row 0 = top, with the p99 auto exposure.
+        Returns (image, exposure, anchor); pass the returned anchor back to lock it."""
+        img = np.zeros((self.ry, self.rx, 3), np.uint8)
+        ex = C.c_double(0)
+        anc = C.c_double(0.0 if anchor is None else float(anchor))
+        self.L.check(self.L.lib.gp_film_tonemap(self.h, int(n_photons), float(exposure_comp), int(bool(absolute)),
+                                                C.byref(anc), img.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                C.byref(ex)))
+        return img, ex.value, anc.value
+
     def close(self):
         if getattr(self, "h", None):
             self.L.lib.gp_film_destroy(self.h)
@@ -187,6 +209,43 @@ class Film:
             self.close()
         except Exception:
             pass
+
+
+def write_pfm(path, lin):
+    """PFM (writePfm, main.cpp:10372-10387): 'PF', width height, -1.0 (little endian),
+    then float32 RGB rows bottom to top -- the row order of `to_linear`."""
+    lin = np.ascontiguousarray(lin, np.float32)
+    h, w = lin.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        f.write(lin.astype("<f4").tobytes())
+
+
+def write_ppm(path, rgb8):
+    """Binary PPM of an 8-bit image whose row 0 is the top."""
+    rgb8 = np.ascontiguousarray(rgb8, np.uint8)
+    h, w = rgb8.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"P6\n{w} {h}\n255\n".encode())
+        f.write(rgb8.tobytes())
+
+
+def write_png(path, rgb8):
+    """8-bit RGB PNG (zlib-compressed scanlines, filter 0) of an image whose row 0 is the top."""
+    import struct
+    import zlib
+    rgb8 = np.ascontiguousarray(rgb8, np.uint8)
+    h, w = rgb8.shape[:2]
+    raw = b"".join(b"\x00" + rgb8[y].tobytes() for y in range(h))
+
+    def chunk(t, d):
+        c = struct.pack(">I", len(d)) + t + d
+        return c + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        f.write(chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)))
+        f.write(chunk(b"IDAT", zlib.compress(raw, 6)))
+        f.write(chunk(b"IEND", b""))
 
 
 def trace_params(n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4, max_depth=32, rr_depth=0):
@@ -253,6 +312,13 @@ class Server:
         n = C.c_int32(0)
         self.L.check(self.L.lib.gp_server_step(self.h, int(max_iterations), C.byref(n)))
         return n.value
+
+    def poll(self, capacity=256):
+        """Ids of finished jobs not yet taken, oldest first."""
+        ids = (C.c_int64 * capacity)()
+        n = C.c_int32(0)
+        self.L.check(self.L.lib.gp_server_poll(self.h, ids, int(capacity), C.byref(n)))
+        return [ids[i] for i in range(n.value)]
 
     def pending(self):
         q, l = C.c_int32(0), C.c_int32(0)

@@ -334,6 +334,21 @@ __device__ __forceinline__ void loadLeaf(const float4* __restrict__ tris, int fi
 // farther pushed with its entry distance; a popped entry farther than the current
 // hit is skipped. Triangles of a leaf are tested in order; a later triangle only
 // replaces the hit if strictly closer.
+//
+// Loop shape (Aila & Laine 2009, "while-while"): the inner loop walks inner nodes and
+// pops in place when a node's children are both missed, so a lane stays in the loop
+// with its wave instead of leaving it to pop; it exits at a leaf or an empty stack.
+// The visit order per ray is the same as one visit per iteration.
+template <int SD, bool SPILL>
+__device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax, int& node) {
+    while (stk.sp > 0) {
+        float t;
+        stk.pop(node, t);
+        if (t <= tmax) return true;
+    }
+    return false;
+}
+
 template <int SD, bool SPILL>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
@@ -342,7 +357,8 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
     int node = 0;
     stk.sp = 0;
     while (true) {
-        if (node >= 0) {
+        bool live = true;
+        while (node >= 0) {
             const float4* nd = nodes + 4 * node;
             const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
@@ -351,37 +367,34 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                 bool lfirst = ph.tl <= ph.tr;
                 stk.push(lfirst ? right : left, lfirst ? ph.tr : ph.tl);
                 node = lfirst ? left : right;
-                continue;
+            } else if (ph.hl) {
+                node = left;
+            } else if (ph.hr) {
+                node = right;
+            } else if (!popNear(stk, tmax, node)) {
+                live = false;
+                break;
             }
-            if (ph.hl) { node = left; continue; }
-            if (ph.hr) { node = right; continue; }
-        } else {
-            const int code = ~node;
-            const int first = code >> 3, count = (code & 7) + 1;
-            LeafTris L;
-            loadLeaf(tris, first, count, L);
+        }
+        if (!live) break;
+        const int code = ~node;
+        const int first = code >> 3, count = (code & 7) + 1;
+        LeafTris L;
+        loadLeaf(tris, first, count, L);
 #pragma unroll
-            for (int k = 0; k < kLeafMax; ++k) {
-                float t;
-                if (k < count && hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t)) {
-                    tmax = t;
-                    best = first + k;
-                }
+        for (int k = 0; k < kLeafMax; ++k) {
+            float t;
+            if (k < count && hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t)) {
+                tmax = t;
+                best = first + k;
             }
         }
-        // pop the next entry that can still hold a closer hit
-        bool found = false;
-        while (stk.sp > 0) {
-            float t;
-            stk.pop(node, t);
-            if (t <= tmax) { found = true; break; }
-        }
-        if (!found) break;
+        if (!popNear(stk, tmax, node)) break;
     }
     return best;
 }
 
-// Any hit on [tmin, tmax).
+// Any hit on [tmin, tmax). Same loop shape as traceClosest.
 template <int SD, bool SPILL>
 __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                                          const float4* __restrict__ tris, const RayPre& r,
@@ -389,7 +402,8 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
     int node = 0;
     stk.sp = 0;
     while (true) {
-        if (node >= 0) {
+        bool live = true;
+        while (node >= 0) {
             const float4* nd = nodes + 4 * node;
             const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
@@ -398,23 +412,30 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                 bool lfirst = ph.tl <= ph.tr;
                 stk.push(lfirst ? right : left, 0.0f);
                 node = lfirst ? left : right;
-                continue;
-            }
-            if (ph.hl) { node = left; continue; }
-            if (ph.hr) { node = right; continue; }
-        } else {
-            const int code = ~node;
-            const int first = code >> 3, count = (code & 7) + 1;
-            LeafTris L;
-            loadLeaf(tris, first, count, L);
-            bool hit = false;
-#pragma unroll
-            for (int k = 0; k < kLeafMax; ++k) {
+            } else if (ph.hl) {
+                node = left;
+            } else if (ph.hr) {
+                node = right;
+            } else if (stk.sp > 0) {
                 float t;
-                if (k < count && !hit) hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
+                stk.pop(node, t);
+            } else {
+                live = false;
+                break;
             }
-            if (hit) return true;
         }
+        if (!live) break;
+        const int code = ~node;
+        const int first = code >> 3, count = (code & 7) + 1;
+        LeafTris L;
+        loadLeaf(tris, first, count, L);
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < kLeafMax; ++k) {
+            float t;
+            if (k < count && !hit) hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
+        }
+        if (hit) return true;
         if (stk.sp == 0) break;
         float t;
         stk.pop(node, t);

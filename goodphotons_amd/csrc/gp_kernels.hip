@@ -724,6 +724,33 @@ __global__ void __launch_bounds__(kTravThreads) k_probe_any(const float4* nodes,
     }
 }
 
+// ------------------------------------------------------------------ job table
+// Installs a job into slot a.slot: its record, photon range, emission state and zeroed
+// ledger rows. A job without photons is complete once maxDepth iterations have run
+// from now, like a job whose last photon was emitted in the current iteration.
+__global__ void __launch_bounds__(256) k_install_job(Queues q, JobInstall a) {
+    Ctrl* c = q.ctrl;
+    const int slot = a.slot;
+    if (threadIdx.x == 0) {
+        const_cast<DevJob*>(q.jobs)[slot] = a.job;
+        c->nextPhoton[slot] = a.next;
+        c->endPhoton[slot] = a.end;
+        c->lastEmitIter[slot] = (a.end > a.next) ? -1 : (int)c->iter;
+        c->active[slot] = 1;
+    }
+    for (int k = threadIdx.x; k < kSegs * kAccRow; k += blockDim.x) a.job.acc[k] = 0ull;
+}
+
+__global__ void __launch_bounds__(kMaxJobs) k_set_order(Queues q, OrderUpdate a) {
+    Ctrl* c = q.ctrl;
+    c->order[threadIdx.x] = a.order[threadIdx.x];
+    if (threadIdx.x == 0) {
+        c->nOrder = a.nOrder;
+        c->nJobs = a.nJobs;
+        if (a.clearSlot >= 0) c->active[a.clearSlot] = 0;
+    }
+}
+
 __global__ void k_fill64(unsigned long long* p, size_t n, unsigned long long v) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -786,6 +813,12 @@ void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, 
 #undef GP_CALL
 }
 #undef GP_BY_STACK
+void launchInstallJob(const Queues& q, const JobInstall& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_install_job, dim3(1), dim3(256), 0, s, q, a);
+}
+void launchSetOrder(const Queues& q, const OrderUpdate& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_order, dim3(1), dim3(kMaxJobs), 0, s, q, a);
+}
 void launchFill64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s) {
     if (n == 0) return;
     size_t blocks = (n + 255) / 256;

@@ -48,6 +48,24 @@ void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, 
                     long long n, float tmin, unsigned char* occ, int2* spill, int spillDepth,
                     int lds, int blocks, int threads, hipStream_t s);
 
+// Stream-ordered job-table updates for a running wavefront. The data travel as
+// kernel arguments (copied at launch), so the host needs no buffer that outlives the
+// call and no synchronisation.
+struct JobInstall {
+    DevJob job;
+    unsigned long long next, end;   // photon range
+    int32_t slot;
+    int32_t pad[3];
+};
+struct OrderUpdate {
+    int32_t order[kMaxJobs];
+    int32_t nOrder, nJobs;
+    int32_t clearSlot;               // job slot to deactivate, or -1
+    int32_t pad;
+};
+void launchInstallJob(const Queues& q, const JobInstall& a, hipStream_t s);
+void launchSetOrder(const Queues& q, const OrderUpdate& a, hipStream_t s);
+
 // Zero-fill of 64-bit words by a kernel (usable inside captured graphs).
 void launchFill64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
 

@@ -39,7 +39,8 @@ DeviceState& device() {
             d.why = "device arch " + d.arch + " is not gfx950";
             return d;
         }
-        if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&d.stream) != hipSuccess) {
+        if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&d.stream) != hipSuccess ||
+            hipStreamCreate(&d.copyStream) != hipSuccess) {
             d.why = "stream creation failed";
             return d;
         }

@@ -61,7 +61,8 @@ struct DeviceState {
     int cus = 0;
     size_t ldsPerBlock = 0;
     std::string name, arch;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // engine work
+    hipStream_t copyStream = nullptr;   // result downloads that overlap engine work
 };
 DeviceState& device();
 gp_status requireDevice();

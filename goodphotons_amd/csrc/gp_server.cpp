@@ -3,14 +3,26 @@
 //
 // A job is (scene, camera, photon budget). Jobs wait in a FIFO until a device job
 // slot and their HBM footprint (film + per-job tables) are free; they are then
-// written into the running wavefront's job table between iterations, and the
+// installed into the running wavefront's job table between iterations, and the
 // scheduling kernel starts feeding their photons into free path slots on the next
 // iteration, alongside the paths of jobs already in flight. A path emitted in
 // iteration e finishes by iteration e + maxDepth - 1, so a job whose last photon
 // was emitted in iteration e is complete once e + maxDepth iterations have run;
-// its film is then final and downloaded. Every photon is seeded by (seed_base + i,
-// salt) of its own job and the film accumulates integers, so a job's film and
-// ledger are bit-identical whether it ran alone or in the mix.
+// its film is then final. Every photon is seeded by (seed_base + i, salt) of its own
+// job and the film accumulates integers, so a job's film and ledger are
+// bit-identical whether it ran alone or in the mix.
+//
+// The host never stalls the device between chunks of iterations:
+//   - job installs and order changes are small kernels on the engine stream whose
+//     data travel as kernel arguments (no host buffer has to outlive a copy);
+//   - films are allocated and cleared stream-ordered (hipMallocAsync + fill kernel);
+//   - after each chunk the control block is copied into a pinned ring slot and an
+//     event recorded; the host reads a snapshot only once its event completes and
+//     keeps at most kInflight chunks ahead of the device;
+//   - a finished job's ledger rows are copied back on the engine stream (before its
+//     slot can be reused) and its film on a second stream that overlaps the next
+//     chunks, into pinned buffers recycled across jobs.
+// A server is meant to be driven from one thread.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -23,6 +35,38 @@
 
 using namespace gp;
 
+namespace {
+
+constexpr int kInflight = 2;          // chunks of iterations queued ahead of the host
+
+// Pinned host buffers recycled by size (film downloads are a few fixed sizes).
+struct PinnedPool {
+    std::multimap<size_t, void*> freeList;
+    std::map<void*, size_t> sizeOf;
+    void* get(size_t n) {
+        auto it = freeList.find(n);
+        if (it != freeList.end()) {
+            void* p = it->second;
+            freeList.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n) != hipSuccess) return nullptr;
+        sizeOf[p] = n;
+        return p;
+    }
+    void put(void* p) {
+        if (p) freeList.emplace(sizeOf[p], p);
+    }
+    ~PinnedPool() {
+        for (auto& kv : sizeOf) hipHostFree(kv.first);
+    }
+};
+
+enum class JobState { Waiting, Live, Retiring, Finished };
+
+}  // namespace
+
 struct gp_server {
     gp_server_config cfg{};
     struct Job {
@@ -32,11 +76,15 @@ struct gp_server {
         int slot = -1;
         int64_t footprint = 0;
         gp_status status = GP_OK;
-        bool finished = false;
+        JobState state = JobState::Waiting;
         double submitMs = 0, startMs = 0, finishMs = 0;
         double energyScale = 1.0;
         gp_energy energy{};
         gp_trace_stats stats{};
+        int64_t installSeq = 0;                  // first chunk that ran with the job installed
+        unsigned long long* accHost = nullptr;   // pinned, kSegs * kAccRow words
+        long long* filmHost = nullptr;           // pinned, res * 4 words
+        hipEvent_t accEv = nullptr, filmEv = nullptr;
     };
     std::map<int64_t, Job> jobs;
     std::deque<int64_t> waiting;
@@ -44,10 +92,37 @@ struct gp_server {
     std::vector<int64_t> order;     // live job ids in admission order
     int64_t nextId = 1;
     int64_t liveBytes = 0;
+    int64_t finished = 0;
     bool engineReady = false;
+    // control-block snapshots of the chunks in flight
+    struct Chunk {
+        Ctrl* snap = nullptr;
+        hipEvent_t ev = nullptr;
+        int64_t seq = 0;
+    };
+    int64_t chunkSeq = 0;           // sequence number of the next chunk
+    std::vector<Chunk> ring;
+    std::deque<int> inflight;       // ring indices, oldest first
+    int ringNext = 0;
+    PinnedPool pool;
+    std::vector<hipEvent_t> eventPool;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    hipEvent_t event() {
+        if (!eventPool.empty()) {
+            hipEvent_t e = eventPool.back();
+            eventPool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        return e;
+    }
+    void recycle(hipEvent_t& e) {
+        if (e) eventPool.push_back(e);
+        e = nullptr;
     }
 };
 
@@ -57,37 +132,47 @@ int64_t jobFootprint(const gp_job_desc& j) {
     return (int64_t)j.camera.res_x * j.camera.res_y * 32 + (int64_t)sizeof(DevJob) + 4096;
 }
 
-// Writes the slot's job record and control fields (stream-ordered between iterations).
-gp_status uploadSlot(Engine& E, int slot) {
-    hipStream_t s = device().stream;
-    Ctrl* h = E.hCtrl;
-    Ctrl* d = E.dCtrl;
-    GP_HIP(hipMemcpyAsync(E.dJobs + slot, E.hJobs + slot, sizeof(DevJob), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&d->nextPhoton[slot], &h->nextPhoton[slot], sizeof(unsigned long long), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&d->endPhoton[slot], &h->endPhoton[slot], sizeof(unsigned long long), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&d->lastEmitIter[slot], &h->lastEmitIter[slot], sizeof(int32_t), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&d->active[slot], &h->active[slot], sizeof(int32_t), hipMemcpyHostToDevice, s));
-    return E.resetJobAcc(slot);
+gp_status uploadOrder(Engine& E, gp_server* srv, int clearSlot) {
+    OrderUpdate u{};
+    u.nOrder = (int32_t)srv->order.size();
+    int maxSlot = 0;
+    for (size_t k = 0; k < srv->order.size(); ++k) {
+        u.order[k] = srv->jobs[srv->order[k]].slot;
+        maxSlot = std::max(maxSlot, u.order[k] + 1);
+    }
+    u.nJobs = maxSlot;
+    u.clearSlot = clearSlot;
+    launchSetOrder(E.q, u, device().stream);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
 }
 
-gp_status uploadOrder(Engine& E, gp_server* srv) {
+// Film whose memory comes from the stream-ordered allocator of the engine stream.
+gp_status streamFilm(const gp_camera_desc& cam, int64_t maxPhotons, gp_film** out) {
     hipStream_t s = device().stream;
-    Ctrl* h = E.hCtrl;
-    h->nOrder = (int32_t)srv->order.size();
-    for (size_t k = 0; k < srv->order.size(); ++k) h->order[k] = srv->jobs[srv->order[k]].slot;
-    int maxSlot = 0;
-    for (size_t k = 0; k < srv->order.size(); ++k) maxSlot = std::max(maxSlot, h->order[k] + 1);
-    h->nJobs = maxSlot;
-    GP_HIP(hipMemcpyAsync(E.dCtrl->order, h->order, sizeof(int32_t) * kMaxJobs, hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&E.dCtrl->nOrder, &h->nOrder, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(&E.dCtrl->nJobs, &h->nJobs, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    // The host copy of `order` must stay unchanged until this copy has run.
-    GP_HIP(hipStreamSynchronize(s));
+    std::unique_ptr<gp_film> f(new gp_film());
+    f->rx = cam.res_x;
+    f->ry = cam.res_y;
+    f->maxPhotons = std::max<int64_t>(1, maxPhotons);
+    const size_t words = (size_t)f->rx * f->ry * 4;
+    hipError_t e = hipMallocAsync((void**)&f->d, words * sizeof(unsigned long long), s);
+    if (e != hipSuccess) return fail(GP_ERR_OOM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    launchFill64(f->d, words, 0ull, s);
+    GP_HIP(hipGetLastError());
+    *out = f.release();
     return GP_OK;
+}
+
+void releaseFilm(gp_film* f, hipStream_t s) {
+    if (!f) return;
+    if (f->d) hipFreeAsync(f->d, s);
+    f->d = nullptr;
+    delete f;
 }
 
 gp_status admit(Engine& E, gp_server* srv) {
     bool changed = false;
+    hipStream_t s = device().stream;
     while (!srv->waiting.empty()) {
         const int64_t id = srv->waiting.front();
         gp_server::Job& J = srv->jobs[id];
@@ -97,66 +182,117 @@ gp_status admit(Engine& E, gp_server* srv) {
         if (slot < 0) break;
         if (srv->liveBytes + J.footprint > srv->cfg.hbm_budget_bytes) break;
         srv->waiting.pop_front();
-        gp_status st = gp_film_create(J.desc.camera.res_x, J.desc.camera.res_y,
-                                      std::max<int64_t>(1, J.desc.params.n_photons), &J.film);
-        if (st == GP_OK) {
-            DevJob dj;
-            st = fillJob(J.desc.scene, J.desc.camera, J.film, J.desc.params, slot, dj);
-            if (st == GP_OK) {
-                E.hJobs[slot] = dj;
-                J.energyScale = dj.energyScale;
-            }
-        }
+        JobInstall a{};
+        gp_status st = streamFilm(J.desc.camera, J.desc.params.n_photons, &J.film);
+        if (st == GP_OK) st = fillJob(J.desc.scene, J.desc.camera, J.film, J.desc.params, slot, a.job);
         if (st != GP_OK) {
             J.status = st;
-            J.finished = true;
+            J.state = JobState::Finished;
             J.startMs = J.finishMs = srv->now();
-            if (J.film) { gp_film_destroy(J.film); J.film = nullptr; }
+            srv->finished++;
+            releaseFilm(J.film, s);
+            J.film = nullptr;
             continue;
         }
-        Ctrl* h = E.hCtrl;
-        h->nextPhoton[slot] = 0;
-        h->endPhoton[slot] = (unsigned long long)J.desc.params.n_photons;
-        h->lastEmitIter[slot] = J.desc.params.n_photons == 0 ? (int32_t)h->iter : -1;
-        h->active[slot] = 1;
-        if ((st = uploadSlot(E, slot)) != GP_OK) return st;
+        J.energyScale = a.job.energyScale;
+        a.next = 0;
+        a.end = (unsigned long long)J.desc.params.n_photons;
+        a.slot = slot;
+        launchInstallJob(E.q, a, s);
+        GP_HIP(hipGetLastError());
         J.slot = slot;
+        J.state = JobState::Live;
+        J.installSeq = srv->chunkSeq;
         J.startMs = srv->now();
         srv->slotJob[(size_t)slot] = id;
         srv->order.push_back(id);
         srv->liveBytes += J.footprint;
         changed = true;
     }
-    if (changed) return uploadOrder(E, srv);
+    if (changed) return uploadOrder(E, srv, -1);
     return GP_OK;
 }
 
-gp_status retire(Engine& E, gp_server* srv, int32_t* finished) {
-    const Ctrl& h = *E.hCtrl;
+// A finished job's results leave the device: ledger rows on the engine stream (so a
+// later install into the slot cannot overtake the copy), film on the copy stream
+// once the chunk that finished it has completed.
+gp_status startRetire(Engine& E, gp_server* srv, gp_server::Job& J, hipEvent_t chunkEv) {
+    hipStream_t s = device().stream, cs = device().copyStream;
+    J.accHost = (unsigned long long*)srv->pool.get(sizeof(unsigned long long) * kSegs * kAccRow);
+    const size_t filmBytes = (size_t)J.film->rx * J.film->ry * 4 * sizeof(long long);
+    J.filmHost = (long long*)srv->pool.get(filmBytes);
+    if (!J.accHost || !J.filmHost) return fail(GP_ERR_OOM, "pinned host buffer allocation failed");
+    J.accEv = srv->event();
+    J.filmEv = srv->event();
+    GP_HIP(hipMemcpyAsync(J.accHost, E.dJobAcc + (size_t)J.slot * kSegs * kAccRow,
+                          sizeof(unsigned long long) * kSegs * kAccRow, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipEventRecord(J.accEv, s));
+    GP_HIP(hipStreamWaitEvent(cs, chunkEv, 0));
+    GP_HIP(hipMemcpyAsync(J.filmHost, J.film->d, filmBytes, hipMemcpyDeviceToHost, cs));
+    GP_HIP(hipEventRecord(J.filmEv, cs));
+    J.state = JobState::Retiring;
+    return GP_OK;
+}
+
+void finishRetire(gp_server* srv, gp_server::Job& J) {
+    unsigned long long ev[5] = {}, cv[4] = {};
+    for (int g = 0; g < kSegs; ++g) {
+        for (int k = 0; k < 5; ++k) ev[k] += J.accHost[g * kAccRow + k];
+        for (int k = 0; k < 4; ++k) cv[k] += J.accHost[g * kAccRow + 5 + k];
+    }
+    J.energy.emitted = (double)ev[0] / J.energyScale;
+    J.energy.absorbed = (double)ev[1] / J.energyScale;
+    J.energy.sensor = (double)ev[2] / J.energyScale;
+    J.energy.escaped = (double)ev[3] / J.energyScale;
+    J.energy.residual = (double)ev[4] / J.energyScale;
+    J.stats.photons = cv[0];
+    J.stats.segments = cv[1];
+    J.stats.connections = cv[2];
+    J.stats.splats = cv[3];
+    srv->pool.put(J.accHost);
+    J.accHost = nullptr;
+    srv->recycle(J.accEv);
+    J.state = JobState::Finished;
+    J.finishMs = srv->now();
+    J.stats.wall_ms = J.finishMs - J.startMs;
+    srv->finished++;
+}
+
+// Finalise retiring jobs whose ledger copy has landed (blocking: wait for them).
+gp_status pollRetiring(gp_server* srv, bool blocking) {
+    for (auto& kv : srv->jobs) {
+        gp_server::Job& J = kv.second;
+        if (J.state != JobState::Retiring) continue;
+        if (blocking) {
+            GP_HIP(hipEventSynchronize(J.accEv));
+        } else {
+            hipError_t q = hipEventQuery(J.accEv);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return fail(GP_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+        }
+        finishRetire(srv, J);
+    }
+    return GP_OK;
+}
+
+// Read the oldest chunk's snapshot (waiting for it) and retire the jobs it shows done.
+gp_status processOldest(Engine& E, gp_server* srv) {
+    const int ri = srv->inflight.front();
+    gp_server::Chunk& c = srv->ring[(size_t)ri];
+    GP_HIP(hipEventSynchronize(c.ev));
+    srv->inflight.pop_front();
+    const Ctrl& h = *c.snap;
+    if (h.overflow) return fail(GP_ERR_CAPACITY, "wavefront queue segment overflow");
     bool changed = false;
     for (size_t k = 0; k < srv->order.size();) {
         const int64_t id = srv->order[k];
         gp_server::Job& J = srv->jobs[id];
-        const int slot = J.slot;
-        const int last = h.lastEmitIter[slot];
+        // a snapshot taken before the job was installed shows the slot's previous job
+        const int last = c.seq >= J.installSeq ? h.lastEmitIter[J.slot] : -1;
         if (last >= 0 && (int64_t)h.iter >= (int64_t)last + J.desc.params.max_depth) {
-            unsigned long long ev[5], cv[4];
-            gp_status st = E.readJobAcc(slot, ev, cv);
+            gp_status st = startRetire(E, srv, J, c.ev);
             if (st != GP_OK) return st;
-            J.energy.emitted = (double)ev[0] / J.energyScale;
-            J.energy.absorbed = (double)ev[1] / J.energyScale;
-            J.energy.sensor = (double)ev[2] / J.energyScale;
-            J.energy.escaped = (double)ev[3] / J.energyScale;
-            J.energy.residual = (double)ev[4] / J.energyScale;
-            J.stats.photons = cv[0];
-            J.stats.segments = cv[1];
-            J.stats.connections = cv[2];
-            J.stats.splats = cv[3];
-            J.finished = true;
-            J.finishMs = srv->now();
-            J.stats.wall_ms = J.finishMs - J.startMs;
-            E.hCtrl->active[slot] = 0;
-            srv->slotJob[(size_t)slot] = -1;
+            srv->slotJob[(size_t)J.slot] = -1;
             srv->liveBytes -= J.footprint;
             srv->order.erase(srv->order.begin() + (long)k);
             changed = true;
@@ -164,14 +300,21 @@ gp_status retire(Engine& E, gp_server* srv, int32_t* finished) {
         }
         ++k;
     }
-    if (changed) {
-        GP_HIP(hipMemcpy(E.dCtrl->active, E.hCtrl->active, sizeof(int32_t) * kMaxJobs, hipMemcpyHostToDevice));
-        gp_status st = uploadOrder(E, srv);
-        if (st != GP_OK) return st;
-    }
-    int32_t n = 0;
-    for (auto& kv : srv->jobs) n += kv.second.finished ? 1 : 0;
-    if (finished) *finished = n;
+    if (changed) return uploadOrder(E, srv, -1);
+    return GP_OK;
+}
+
+gp_status launchChunk(Engine& E, gp_server* srv, int K) {
+    hipStream_t s = device().stream;
+    gp_status st = E.runIterations(K);
+    if (st != GP_OK) return st;
+    const int ri = srv->ringNext;
+    srv->ringNext = (srv->ringNext + 1) % (int)srv->ring.size();
+    gp_server::Chunk& c = srv->ring[(size_t)ri];
+    GP_HIP(hipMemcpyAsync(c.snap, E.dCtrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipEventRecord(c.ev, s));
+    c.seq = srv->chunkSeq++;
+    srv->inflight.push_back(ri);
     return GP_OK;
 }
 
@@ -189,6 +332,11 @@ gp_status gp_server_create(const gp_server_config* cfg, gp_server** out) {
     std::unique_ptr<gp_server> s(new gp_server());
     s->cfg = *cfg;
     s->slotJob.assign((size_t)cfg->max_live_jobs, -1);
+    s->ring.resize(kInflight + 1);
+    for (auto& c : s->ring) {
+        GP_HIP(hipHostMalloc((void**)&c.snap, sizeof(Ctrl)));
+        GP_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+    }
     *out = s.release();
     return GP_OK;
 }
@@ -224,10 +372,21 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     int maxDepthScene = 0;
     for (auto& kv : srv->jobs) maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
     const int lds = E.tun.lds_stack;
+    if (srv->engineReady && !srv->inflight.empty()) {
+        // a change of engine shape would re-allocate under running chunks
+        const int need = std::max(0, maxDepthScene + 1 - lds);
+        if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0)) {
+            while (!srv->inflight.empty())
+                if ((st = processOldest(E, srv)) != GP_OK) return st;
+            GP_HIP(hipStreamSynchronize(device().stream));
+        }
+    }
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds))) != GP_OK) return st;
     const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg;
     if (fresh) {
+        if (!srv->order.empty() || !srv->inflight.empty())
+            return fail(GP_ERR_STATE, "engine re-allocated under a running server");
         E.initCtrl();
         GP_HIP(hipMemcpy(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice));
         E.parity = 0;
@@ -236,17 +395,27 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     const int K = std::max(2, E.tun.iters_per_graph & ~1);
     int64_t ran = 0;
     while (true) {
+        if ((st = pollRetiring(srv, false)) != GP_OK) return st;
         if ((st = admit(E, srv)) != GP_OK) return st;
-        if (srv->order.empty()) break;
-        if ((st = E.runIterations(K)) != GP_OK) return st;
-        ran += K;
-        if ((st = E.readCtrl()) != GP_OK) return st;
-        if ((st = retire(E, srv, finishedOut)) != GP_OK) return st;
+        const bool live = !srv->order.empty();
+        if (!live && srv->inflight.empty()) break;
+        if (live && (int)srv->inflight.size() < kInflight) {
+            if ((st = launchChunk(E, srv, K)) != GP_OK) return st;
+            ran += K;
+        } else {
+            if ((st = processOldest(E, srv)) != GP_OK) return st;
+        }
+        // everything else that has already completed, without waiting
+        while (!srv->inflight.empty()) {
+            hipError_t q = hipEventQuery(srv->ring[(size_t)srv->inflight.front()].ev);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) return fail(GP_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+            if ((st = processOldest(E, srv)) != GP_OK) return st;
+        }
         if (maxIterations > 0 && ran >= maxIterations) break;
     }
-    int32_t n = 0;
-    for (auto& kv : srv->jobs) n += kv.second.finished ? 1 : 0;
-    if (finishedOut) *finishedOut = n;
+    if (maxIterations <= 0 && (st = pollRetiring(srv, true)) != GP_OK) return st;
+    if (finishedOut) *finishedOut = (int32_t)srv->finished;
     return GP_OK;
 }
 
@@ -256,7 +425,11 @@ gp_status gp_server_take_result(gp_server* srv, int64_t jobId, gp_job_result* re
     auto it = srv->jobs.find(jobId);
     if (it == srv->jobs.end()) return fail(GP_ERR_INVALID, "unknown job id");
     gp_server::Job& J = it->second;
-    if (!J.finished) return fail(GP_ERR_STATE, "job not finished");
+    if (J.state == JobState::Retiring) {
+        GP_HIP(hipEventSynchronize(J.accEv));
+        finishRetire(srv, J);
+    }
+    if (J.state != JobState::Finished) return fail(GP_ERR_STATE, "job not finished");
     if (res) {
         std::memset(res, 0, sizeof *res);
         res->job_id = J.id;
@@ -270,14 +443,46 @@ gp_status gp_server_take_result(gp_server* srv, int64_t jobId, gp_job_result* re
         res->start_ms = J.startMs;
         res->finish_ms = J.finishMs;
     }
-    gp_status st = GP_OK;
     if (J.film) {
-        if (xyz || hits) st = gp_film_download(J.film, xyz, hits);
-        gp_film_destroy(J.film);
+        if (J.filmEv) GP_HIP(hipEventSynchronize(J.filmEv));
+        if (J.filmHost && (xyz || hits)) {
+            const size_t np = (size_t)J.film->rx * J.film->ry;
+            const double inv = J.film->scale > 0 ? 1.0 / J.film->scale : 0.0;
+            const long long* w = J.filmHost;
+            for (size_t i = 0; i < np; ++i) {
+                if (xyz) {
+                    xyz[3 * i + 0] = (double)w[4 * i + 0] * inv;
+                    xyz[3 * i + 1] = (double)w[4 * i + 1] * inv;
+                    xyz[3 * i + 2] = (double)w[4 * i + 2] * inv;
+                }
+                if (hits) hits[i] = (double)w[4 * i + 3];
+            }
+        }
+        srv->pool.put(J.filmHost);
+        J.filmHost = nullptr;
+        srv->recycle(J.filmEv);
+        releaseFilm(J.film, device().copyStream);
         J.film = nullptr;
     }
     srv->jobs.erase(it);
-    return st;
+    return GP_OK;
+}
+
+gp_status gp_server_poll(gp_server* srv, int64_t* ids, int32_t capacity, int32_t* nOut) {
+    if (!srv || (capacity > 0 && !ids)) return fail(GP_ERR_INVALID, "null argument");
+    gp_status st = pollRetiring(srv, false);
+    if (st != GP_OK) return st;
+    std::vector<std::pair<double, int64_t>> done;
+    for (auto& kv : srv->jobs)
+        if (kv.second.state == JobState::Finished) done.emplace_back(kv.second.finishMs, kv.first);
+    std::sort(done.begin(), done.end());
+    int32_t n = 0;
+    for (auto& d : done) {
+        if (n >= capacity) break;
+        ids[n++] = d.second;
+    }
+    if (nOut) *nOut = n;
+    return GP_OK;
 }
 
 gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live) {
@@ -289,17 +494,28 @@ gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live) {
 
 void gp_server_destroy(gp_server* srv) {
     if (!srv) return;
-    for (auto& kv : srv->jobs)
-        if (kv.second.film) gp_film_destroy(kv.second.film);
-    // Leave no job active in the shared engine.
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
-    if (E.hCtrl && E.dCtrl) {
-        for (int k = 0; k < kMaxJobs; ++k) E.hCtrl->active[k] = 0;
-        E.hCtrl->nOrder = 0;
-        hipMemcpy(E.dCtrl->active, E.hCtrl->active, sizeof(int32_t) * kMaxJobs, hipMemcpyHostToDevice);
-        hipMemcpy(&E.dCtrl->nOrder, &E.hCtrl->nOrder, sizeof(int32_t), hipMemcpyHostToDevice);
+    hipStreamSynchronize(device().stream);
+    hipStreamSynchronize(device().copyStream);
+    // Leave no job in the shared engine's emission order.
+    if (srv->engineReady && E.dCtrl) {
+        srv->order.clear();
+        uploadOrder(E, srv, -1);
+        hipStreamSynchronize(device().stream);
     }
+    for (auto& kv : srv->jobs) {
+        gp_server::Job& J = kv.second;
+        if (J.film) releaseFilm(J.film, device().stream);
+        if (J.accEv) hipEventDestroy(J.accEv);
+        if (J.filmEv) hipEventDestroy(J.filmEv);
+    }
+    hipStreamSynchronize(device().stream);
+    for (auto& c : srv->ring) {
+        if (c.snap) hipHostFree(c.snap);
+        if (c.ev) hipEventDestroy(c.ev);
+    }
+    for (hipEvent_t e : srv->eventPool) hipEventDestroy(e);
     delete srv;
 }
 

@@ -171,10 +171,10 @@ def run_c4(n_jobs=60, lam=20.0, full=True):
         segs_plain += st["segments"]
         lat_plain.append(time.perf_counter() - t0 - j["arrival"])
     make_plain = time.perf_counter() - t0
-    # continuous batching
+    # continuous batching: submit on arrival, keep the device fed, take results as
+    # they finish (films downloaded on both sides)
     srv = gp.Server(hbm_budget_bytes=8 << 30, max_live_jobs=64, lib=lib)
     t0 = time.perf_counter()
-    pending = list(range(len(jobs)))
     live = {}
     lat_srv, segs_srv = [None] * len(jobs), 0
     nxt = 0
@@ -192,19 +192,11 @@ def run_c4(n_jobs=60, lam=20.0, full=True):
             time.sleep(max(0.0, jobs[nxt]["arrival"] - (time.perf_counter() - t0)))
             continue
         srv.step(max_iterations=8)
-        done_ids = []
-        for jid, (idx, res) in live.items():
-            try:
-                r, _, _ = srv.take(jid, res, res, film=False)
-            except gp.GpError as ex:
-                if ex.status == 8:     # not finished yet
-                    continue
-                raise
+        for jid in srv.poll():
+            idx, res = live.pop(jid)
+            r, x, h = srv.take(jid, res, res, film=True)
             lat_srv[idx] = time.perf_counter() - t0 - jobs[idx]["arrival"]
             segs_srv += r.stats.segments
-            done_ids.append(jid)
-        for jid in done_ids:
-            del live[jid]
     make_srv = time.perf_counter() - t0
     srv.close()
     span_plain = make_plain - jobs[0]["arrival"]

@@ -25,6 +25,10 @@
  *                             (src/render_cuda.h:446-464)
  *   gp_probe_any_hit       <- Scene::occluded / Bvh::traverseAny
  *                             (src/scene.h:2249, src/bvh.h:228-249)
+ *   gp_film_to_linear      <- filmToLinear (src/main.cpp:10282-10294) feeding
+ *                             writePfm (src/main.cpp:10372-10387)
+ *   gp_film_tonemap        <- filmToRgb8 (src/main.cpp:10296-10348): p99
+ *                             auto-exposure and 8-bit sRGB, computed on the device
  *   gp_server_*            <- no single counterpart: the resident session
  *                             (sharedForwardGpuBegin/Batch/Download/End,
  *                             src/render_cuda.h:121-143) generalised to many
@@ -213,6 +217,23 @@ gp_status gp_film_download(gp_film* film, double* xyz, double* hits);
 gp_status gp_film_download_raw(gp_film* film, int64_t* words, double* scale_out);
 void gp_film_destroy(gp_film* film);
 
+/* ---- film -> image ------------------------------------------------------- */
+/* Scene-linear sRGB radiance of a film that holds n_photons photons:
+ * rgb = xyzToLinearSrgb(xyz / (n_photons * integral(ybar))) per pixel, in double,
+ * stored as float32 triples in the film's row order (row 0 = bottom, the PFM
+ * raster order). rgb holds res_x*res_y*3 floats. Replaces filmToLinear
+ * (src/main.cpp:10282-10294; the optional denoiser is not part of this core). */
+gp_status gp_film_to_linear(gp_film* film, int64_t n_photons, float* rgb);
+/* 8-bit sRGB image (res_x*res_y*3 bytes, row 0 = image top) as filmToRgb8
+ * (src/main.cpp:10296-10348): exposure = e_auto * (exposure_comp > 0 ? exposure_comp
+ * : 1) with e_auto = 6 when `absolute`, else *anchor when anchor && *anchor > 0, else
+ * 0.9 / p99 of max(r, g, b, 0) over the pixels (the element at index
+ * (size_t)(0.99 (n - 1)) in order; 1 if p99 <= 0), which is then stored to *anchor
+ * when anchor is non-null. The reduction runs on the device; only the bytes come
+ * back. exposure_out (optional) receives the exposure used. */
+gp_status gp_film_tonemap(gp_film* film, int64_t n_photons, double exposure_comp, int32_t absolute,
+                          double* anchor, uint8_t* rgb8, double* exposure_out);
+
 /* ---- tracing ------------------------------------------------------------ */
 /* Trace params->n_photons photons of `scene` through `cam` into `film`
  * (blocking). Energy is ACCUMULATED into *energy (+=), as renderForwardCuda
@@ -277,9 +298,15 @@ gp_status gp_server_create(const gp_server_config* cfg, gp_server** out);
 /* Queue a job. It is admitted into the running wavefront as soon as a job slot
  * and its HBM footprint are free. Returns its id in *job_id. */
 gp_status gp_server_submit(gp_server* srv, const gp_job_desc* job, int64_t* job_id);
-/* Advance the wavefront by at most max_iterations iterations (0 = until every
- * queued job has finished). Returns the number of jobs finished so far. */
+/* Advance the wavefront: queue at least max_iterations more iterations and return
+ * (0 = run until every queued job has finished and its results are on the host).
+ * With max_iterations > 0 the call returns while up to two chunks of iterations are
+ * still running, so the caller's work overlaps the device; the next step, poll or
+ * take picks up where it left. Returns the number of jobs finished so far. */
 gp_status gp_server_step(gp_server* srv, int32_t max_iterations, int32_t* finished_out);
+/* Ids of finished jobs not yet taken, oldest first (at most `capacity`), without
+ * waiting for the device. n_out receives the count written. */
+gp_status gp_server_poll(gp_server* srv, int64_t* ids_out, int32_t capacity, int32_t* n_out);
 /* Fetch a finished job's result and film (xyz res*3, hits res, either may be
  * null). Returns GP_ERR_STATE if the job is not finished. The job's device
  * film is released after this call. */

@@ -78,6 +78,11 @@ class _CpuRenderer:
         f.argtypes = [C.POINTER(abi.CameraDesc), _dp]
         f = getattr(L, p + "camera_project"); f.restype = C.c_int
         f.argtypes = [C.POINTER(abi.CameraDesc), _dp, C.c_int, _i32p, _i32p, _dp, _dp, _dp]
+        if p == "or_":   # film -> image: restated from main.cpp (no reference entry point)
+            f = L.or_film_to_linear; f.restype = None
+            f.argtypes = [_dp, C.c_int64, C.c_double, _dp]
+            f = L.or_film_to_rgb8; f.restype = None
+            f.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _u8p, _dp, _dp]
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -114,6 +119,25 @@ class _CpuRenderer:
 
     def cie_y_integral(self):
         return float(self._f("cie_y_integral")())
+
+    # ---- film -> image (oracle only)
+    def film_to_linear(self, xyz, n_photons):
+        """xyz: (npix, 3) film sums in row order -> (npix, 3) linear sRGB (double)."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        out = np.zeros_like(xyz)
+        self.lib.or_film_to_linear(_ptr(xyz, _dp), len(xyz), float(n_photons), _ptr(out, _dp))
+        return out
+
+    def film_to_rgb8(self, xyz, w, h, n_photons, exp_comp=0.0, absolute=False, anchor=None):
+        """8-bit sRGB (h, w, 3), row 0 = image top; returns (img, e_auto, exposure, anchor)."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        img = np.zeros((h, w, 3), np.uint8)
+        ea, ex = C.c_double(), C.c_double()
+        anc = None if anchor is None else C.c_double(anchor)
+        self.lib.or_film_to_rgb8(_ptr(xyz, _dp), w, h, float(n_photons), float(exp_comp), int(bool(absolute)),
+                                 C.byref(anc) if anc is not None else None, _ptr(img, _u8p),
+                                 C.byref(ea), C.byref(ex))
+        return img, ea.value, ex.value, (None if anc is None else anc.value)
 
     def emission_sample(self, spec, u):
         u = np.ascontiguousarray(u, np.float64)

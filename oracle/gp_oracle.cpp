@@ -16,7 +16,9 @@
 //   2005-2044, 2194-2271 (lights, build, closestHit, occluded), camera.h:110-240,
 //   render.h:46-64 (glossy lobe), :499-577 (connectGeom, connectHero),
 //   :1612-1733 (specular interactions), :2252-2631 (tracePhotonHero + loop),
-//   :2657-2694 (refractOrReflect), main.cpp:11305-11340 (thread pool + seeding).
+//   :2657-2694 (refractOrReflect), main.cpp:11305-11340 (thread pool + seeding),
+//   main.cpp:10265-10348 + color.h:41-53 (filmToLinear, filmToRgb8, xyzToLinearSrgb,
+//   srgbGamma; without the optional denoiser).
 #include "gp_oracle.hpp"
 
 #include <algorithm>
@@ -1062,6 +1064,66 @@ int or_camera_project(const gp_camera_desc* cd, const double* pts, int n, int32_
         omega[i] = ok ? c.pixelOmega(cc) : 0.0;
     }
     return 0;
+}
+
+
+// ---- film -> image (src/main.cpp:10282-10348, src/color.h:41-53)
+// filmToLinear without the optional denoiser: norm = 1 / (N * cieYIntegral()), each
+// pixel lin = xyzToLinearSrgb(xyz * norm), in double.
+void or_film_to_linear(const double* xyz, int64_t npix, double N, double* lin) {
+    const double norm = 1.0 / (N * ybarIntegral());
+    for (int64_t i = 0; i < npix; ++i) {
+        const double x = xyz[3 * i + 0] * norm, y = xyz[3 * i + 1] * norm, z = xyz[3 * i + 2] * norm;
+        lin[3 * i + 0] = 3.2406 * x - 1.5372 * y - 0.4986 * z;
+        lin[3 * i + 1] = -0.9689 * x + 1.8758 * y + 0.0415 * z;
+        lin[3 * i + 2] = 0.0557 * x - 0.2040 * y + 1.0570 * z;
+    }
+}
+
+static double srgbGammaD(double c) {
+    c = std::max(0.0, c);
+    return (c <= 0.0031308) ? 12.92 * c : 1.055 * std::pow(c, 1.0 / 2.4) - 0.055;
+}
+
+// filmToRgb8: exposure = eAuto * (expComp > 0 ? expComp : 1) with eAuto = 6 (absolute
+// EV), the locked anchor, or 0.9 / p99 where p99 is the element at (size_t)(0.99 (n-1))
+// of max(r, g, b, 0) over the pixels (1 when p99 <= 0); 8-bit sRGB, rows flipped so
+// row 0 is the image top.
+void or_film_to_rgb8(const double* xyz, int W, int H, double N, double expComp, int absolute,
+                     double* lockAnchor, uint8_t* img, double* eAutoOut, double* exposureOut) {
+    const size_t n = (size_t)W * H;
+    std::vector<double> lin(3 * n);
+    or_film_to_linear(xyz, (int64_t)n, N, lin.data());
+    double eAuto, exposure;
+    if (absolute) {
+        eAuto = 6.0;
+        exposure = eAuto * (expComp > 0.0 ? expComp : 1.0);
+    } else {
+        if (lockAnchor && *lockAnchor > 0.0) {
+            eAuto = *lockAnchor;
+        } else {
+            std::vector<double> lum(n);
+            for (size_t i = 0; i < n; ++i)
+                lum[i] = std::max({lin[3 * i], lin[3 * i + 1], lin[3 * i + 2], 0.0});
+            const size_t k = (size_t)(0.99 * (double)(n - 1));
+            std::nth_element(lum.begin(), lum.begin() + (long)k, lum.end());
+            const double p99 = lum[k];
+            eAuto = (p99 > 0) ? 0.9 / p99 : 1.0;
+            if (lockAnchor) *lockAnchor = eAuto;
+        }
+        exposure = eAuto * (expComp > 0.0 ? expComp : 1.0);
+    }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t src = (size_t)(H - 1 - y) * W + x;
+            const size_t dst = ((size_t)y * W + x) * 3;
+            for (int c = 0; c < 3; ++c) {
+                const double v = lin[3 * src + c] * exposure;
+                img[dst + c] = (uint8_t)std::clamp(srgbGammaD(v) * 255.0 + 0.5, 0.0, 255.0);
+            }
+        }
+    if (eAutoOut) *eAutoOut = eAuto;
+    if (exposureOut) *exposureOut = exposure;
 }
 
 }  // extern "C"

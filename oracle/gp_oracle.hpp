@@ -43,6 +43,9 @@ uint64_t or_mix64(uint64_t x);
 void or_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out);
 void or_cie(const double* lam, int n, double* xyz);
 double or_cie_y_integral();
+void or_film_to_linear(const double* xyz, int64_t npix, double N, double* lin);
+void or_film_to_rgb8(const double* xyz, int W, int H, double N, double expComp, int absolute,
+                     double* lockAnchor, uint8_t* img, double* eAutoOut, double* exposureOut);
 void or_emission_sample(const gp_spectrum* s, const double* u, int n, double* lam, double* pdf,
                         double* integral);
 int or_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double* pdf);

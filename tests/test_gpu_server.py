@@ -69,7 +69,9 @@ def test_late_arrivals_join_running_wavefront(gplib):
     late = [srv.submit(gs, sd.camera, 1 << 14, seed_base=1000 * k, max_depth=8) for k in range(1, 6)]
     srv.step()
     assert srv.pending() == (0, 0)
+    assert sorted(srv.poll()) == sorted([first] + late)
     res = [srv.take(j, 32, 32, film=False)[0] for j in [first] + late]
+    assert srv.poll() == []
     assert all(r.status == 0 for r in res)
     assert all(r.stats.photons == n for r, n in zip(res, [1 << 23] + [1 << 14] * 5))
     assert all(r.finish_ms >= r.start_ms >= r.submit_ms for r in res)
