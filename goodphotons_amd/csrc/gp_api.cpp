@@ -400,7 +400,7 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
         GP_HIP(hipMemcpyAsync(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, strm));
         if ((st = E.resetJobAcc(0)) != GP_OK) return st;
         E.parity = 0;
-        const int K = std::max(2, E.tun.iters_per_graph & ~1);
+        const int K = chunkIterations(E.tun);
         // Every iteration retires or advances each live path by one segment, so the
         // loop needs at most ceil(n / capacity) refills times maxDepth iterations.
         const int64_t maxIters = ((n + E.capacity - 1) / E.capacity + 2) * (int64_t)(p->max_depth + 2) * 2 + 64;
@@ -526,7 +526,8 @@ gp_status gp_set_tuning(const gp_tuning* in) {
     const int q = in->queue_capacity;
     if (q < 4096 || q > (1 << 24) || (q & (q - 1)) != 0) return fail(GP_ERR_INVALID, "queue_capacity must be a power of two in [2^12, 2^24]");
     if (in->grid_waves_per_cu < 4 || in->grid_waves_per_cu > 64) return fail(GP_ERR_INVALID, "grid_waves_per_cu must be 4..64");
-    if (in->iters_per_graph < 2 || in->iters_per_graph > 64) return fail(GP_ERR_INVALID, "iters_per_graph must be 2..64");
+    if (in->iters_per_graph < 1 || in->iters_per_graph > 64 || (in->use_graph && in->iters_per_graph < 2))
+        return fail(GP_ERR_INVALID, "iters_per_graph must be 2..64 (1..64 without graphs)");
     if (in->lds_stack != 8 && in->lds_stack != 16 && in->lds_stack != 24 && in->lds_stack != 32)
         return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
     Engine& E = engine();

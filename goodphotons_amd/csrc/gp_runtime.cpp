@@ -485,7 +485,7 @@ gp_status Engine::collectProfile() {
 
 gp_status Engine::runIterations(int n) {
     hipStream_t s = device().stream;
-    const int K = std::max(2, tun.iters_per_graph & ~1);
+    const int K = chunkIterations(tun);
     if (!tun.use_graph || tun.profile || parity != 0 || n < K) return enqueueIterations(n);
     if (!graph || graphIters != K || graphLds != q.lds) {
         if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }

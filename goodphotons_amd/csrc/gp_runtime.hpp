@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <mutex>
 #include <string>
@@ -121,6 +122,11 @@ struct Engine {
 };
 Engine& engine();
 gp_tuning defaultTuning();
+// Iterations between host reads of the control block: iters_per_graph, rounded down
+// to even when graphs are replayed (a graph covers both queue parities).
+inline int chunkIterations(const gp_tuning& t) {
+    return t.use_graph ? std::max(2, t.iters_per_graph & ~1) : std::max(1, t.iters_per_graph);
+}
 unsigned segmentBound(unsigned cap, unsigned blocks);
 
 gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,

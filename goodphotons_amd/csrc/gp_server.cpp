@@ -392,7 +392,7 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
         E.parity = 0;
         srv->engineReady = true;
     }
-    const int K = std::max(2, E.tun.iters_per_graph & ~1);
+    const int K = chunkIterations(E.tun);
     int64_t ran = 0;
     while (true) {
         if ((st = pollRetiring(srv, false)) != GP_OK) return st;

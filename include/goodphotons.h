@@ -257,7 +257,8 @@ typedef struct gp_tuning {
     int32_t block_extend;    /* threads per block of the traversal kernels           */
     int32_t block_shade;     /* threads per block of emit/shade                      */
     int32_t grid_waves_per_cu; /* resident waves per CU targeted by the fixed grids  */
-    int32_t iters_per_graph; /* wavefront iterations per captured hipGraph           */
+    int32_t iters_per_graph; /* iterations per host read of the wavefront state (per
+                                captured hipGraph: even, 2..64; 1..64 without graphs) */
     int32_t use_graph;       /* 1 = replay captured graphs, 0 = plain launches       */
     int32_t lds_stack;       /* traversal stack entries per lane held in LDS (8..32)  */
     int32_t profile;         /* 1 = time every kernel launch with HIP events (no graphs) */
