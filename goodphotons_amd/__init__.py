@@ -53,6 +53,13 @@ class Library:
         L.gp_film_destroy.argtypes = [vp]
         L.gp_film_destroy.restype = None
         L.gp_film_to_linear.argtypes = [vp, C.c_int64, C.POINTER(C.c_float)]
+        L.gp_checkpoint_guard.restype = C.c_uint64
+        L.gp_checkpoint_guard.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int32,
+                                          C.c_int32, C.c_int32]
+        L.gp_film_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_int64, C.POINTER(abi.Energy), C.c_int32,
+                                              C.c_uint64]
+        L.gp_film_load_checkpoint.argtypes = [C.c_char_p, C.c_int32, C.c_uint64, C.POINTER(vp),
+                                              C.POINTER(C.c_int64), C.POINTER(abi.Energy)]
         L.gp_film_tonemap.argtypes = [vp, C.c_int64, C.c_double, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint8), C.POINTER(C.c_double)]
         L.gp_trace.argtypes = [vp, C.POINTER(abi.CameraDesc), vp, C.POINTER(abi.TraceParams),
@@ -180,6 +187,28 @@ class Film:
 
     def reset(self):
         self.L.check(self.L.lib.gp_film_reset(self.h))
+
+    def save_checkpoint(self, path, n_photons, energy=None, mode="B", guard=0):
+        """FTBUF01 checkpoint (+ exact fixed-point section) of this film."""
+        e = energy if isinstance(energy, abi.Energy) else None
+        self.L.check(self.L.lib.gp_film_save_checkpoint(self.h, os.fsencode(path), int(n_photons),
+                                                        C.byref(e) if e is not None else None,
+                                                        ord(mode), int(guard)))
+
+    @classmethod
+    def load_checkpoint(cls, path, mode="B", guard=0, lib: "Library | None" = None):
+        """(film, cumulative photons, energy) from a checkpoint written by save_checkpoint."""
+        L = lib or load()
+        h = C.c_void_p()
+        n = C.c_int64(0)
+        e = abi.Energy()
+        L.check(L.lib.gp_film_load_checkpoint(os.fsencode(path), ord(mode), int(guard), C.byref(h),
+                                              C.byref(n), C.byref(e)))
+        with open(path, "rb") as fh:
+            rx, ry = np.frombuffer(fh.read(16)[8:16], "<i4")
+        f = cls.__new__(cls)
+        f.L, f.h, f.rx, f.ry = L, h, int(rx), int(ry)
+        return f, n.value, e
 
     def to_linear(self, n_photons):
         """Scene-linear sRGB, float32 (res_y, res_x, 3), row 0 = bottom (PFM order)."""

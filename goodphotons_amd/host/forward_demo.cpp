@@ -1,6 +1,7 @@
 // C++ caller of the forward core through gp_forward.hpp: renders an empty Cornell
 // box (5 wall quads + ceiling light quad) with renderForwardGpu, then again through
-// a ForwardSession in two batches, and writes both films and ledgers as raw
+// a ForwardSession in two batches, and once more checkpointed after the first batch
+// and resumed in a second session; writes the films and ledgers as raw
 // doubles. Usage: forward_demo <out-prefix> <res> <photons>
 #include <cstdio>
 #include <cstdlib>
@@ -86,6 +87,17 @@ int main(int argc, char** argv) {
     gp::Film f2;
     gp::EnergyReport e2;
     if (!sess.download(f2, e2)) return 1;
+    // checkpoint after the first half, resume in a new session, finish there
+    gp::ForwardSession a, b;
+    const uint64_t guard = gp_checkpoint_guard((int64_t)s.tris.size(), 0, (int64_t)s.emitters.size(), 1.0, 'B', res, res);
+    const std::string ck = out + ".ftbuf";
+    if (!a.begin(s, cam, res, res, 4, N) || !a.batch(N / 2, 0) || !a.saveCheckpoint(ck, N / 2, guard)) return 1;
+    if (!b.begin(s, cam, res, res, 4, N)) return 1;
+    const long long done = b.resume(ck, guard);
+    if (done != N / 2 || !b.batch(N - done, (unsigned long long)done)) return 1;
+    gp::Film f4;
+    gp::EnergyReport e4;
+    if (!b.download(f4, e4)) return 1;
     gp::EnergyReport e3;
     gp::Film f3 = gp::renderForwardGpu(s, cam, res, res, N, e3, true, 'A');   // unsupported model
     double sum3 = 0;
@@ -93,6 +105,8 @@ int main(int argc, char** argv) {
     std::printf("emitted %.6e absorbed %.6e escaped %.6e residual %.6e\n", e1.emitted, e1.absorbed,
                 e1.escaped, e1.residual);
     std::printf("unsupported_model_empty %d\n", (int)(sum3 == 0.0 && (int)f3.xyz.size() == res * res));
-    if (!save(out + "_single.bin", f1, e1) || !save(out + "_session.bin", f2, e2)) return 1;
+    if (!save(out + "_single.bin", f1, e1) || !save(out + "_session.bin", f2, e2) ||
+        !save(out + "_resumed.bin", f4, e4))
+        return 1;
     return 0;
 }

@@ -217,6 +217,31 @@ public:
         e = EnergyReport{energy_.emitted, energy_.absorbed, energy_.sensor, energy_.escaped, energy_.residual};
         return true;
     }
+    // Resumable checkpoint of the session film (writeCheckpoint's .ftbuf, plus the
+    // exact fixed-point words). `photons` = photons traced so far; guard as
+    // checkpointGuard(scene, mode, res, resY).
+    bool saveCheckpoint(const std::string& path, long long photons, uint64_t guard) const {
+        if (gp_film_save_checkpoint(film_, path.c_str(), photons, &energy_, 'B', guard) != GP_OK) {
+            detail::report("checkpoint");
+            return false;
+        }
+        return true;
+    }
+    // Replace the session film and ledger with a checkpoint's; returns the photons it
+    // holds (the seedBase of the next batch), or -1.
+    long long resume(const std::string& path, uint64_t guard) {
+        gp_film* f = nullptr;
+        int64_t n = 0;
+        gp_energy e{};
+        if (gp_film_load_checkpoint(path.c_str(), 'B', guard, &f, &n, &e) != GP_OK) {
+            detail::report("resume");
+            return -1;
+        }
+        if (film_) gp_film_destroy(film_);
+        film_ = f;
+        energy_ = e;
+        return n;
+    }
     void end() {
         if (film_) gp_film_destroy(film_);
         if (scene_) gp_scene_destroy(scene_);

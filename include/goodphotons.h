@@ -25,6 +25,9 @@
  *                             (src/render_cuda.h:446-464)
  *   gp_probe_any_hit       <- Scene::occluded / Bvh::traverseAny
  *                             (src/scene.h:2249, src/bvh.h:228-249)
+ *   gp_film_save/load_checkpoint, gp_checkpoint_guard
+ *                          <- writeCheckpoint / readCheckpoint / checkpointGuard
+ *                             (src/main.cpp:12317-12329, :12443-12483)
  *   gp_film_to_linear      <- filmToLinear (src/main.cpp:10282-10294) feeding
  *                             writePfm (src/main.cpp:10372-10387)
  *   gp_film_tonemap        <- filmToRgb8 (src/main.cpp:10296-10348): p99
@@ -216,6 +219,27 @@ gp_status gp_film_download(gp_film* film, double* xyz, double* hits);
  * factor that maps a double contribution to one integer unit. */
 gp_status gp_film_download_raw(gp_film* film, int64_t* words, double* scale_out);
 void gp_film_destroy(gp_film* film);
+
+/* ---- checkpoints (.ftbuf) ------------------------------------------------ */
+/* Identity guard of a checkpoint, as checkpointGuard (src/main.cpp:12317-12329):
+ * the scene's triangle, sphere and emitter counts, the bits of its total emitted
+ * power, the render mode character and the resolution. */
+uint64_t gp_checkpoint_guard(int64_t n_tris, int64_t n_spheres, int64_t n_emitters, double total_power,
+                             int32_t mode, int32_t res_x, int32_t res_y);
+/* Write `film` as an FTBUF01 checkpoint (src/main.cpp:12443-12462) that the
+ * reference's readCheckpoint accepts (n_photons = cumulative photons in the film,
+ * energy = cumulative ledger, may be null), followed by a tagged section with the
+ * exact fixed-point words, so gp_film_load_checkpoint resumes bit-exactly. Replaces
+ * writeCheckpoint for a film resident on the device. */
+gp_status gp_film_save_checkpoint(gp_film* film, const char* path, int64_t n_photons,
+                                  const gp_energy* energy, int32_t mode, uint64_t guard);
+/* Recreate a device film from a checkpoint written by gp_film_save_checkpoint.
+ * GP_ERR_STATE if mode or guard differ from the file's (readCheckpoint refuses the
+ * same files); GP_ERR_UNSUPPORTED for a file without the fixed-point section (one
+ * the reference wrote: resume it on the host, as readCheckpoint + Film::merge do).
+ * n_photons_out: the cumulative photon count, i.e. the seed_base of the next trace. */
+gp_status gp_film_load_checkpoint(const char* path, int32_t mode, uint64_t guard, gp_film** film_out,
+                                  int64_t* n_photons_out, gp_energy* energy_out);
 
 /* ---- film -> image ------------------------------------------------------- */
 /* Scene-linear sRGB radiance of a film that holds n_photons photons:

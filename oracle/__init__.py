@@ -81,6 +81,10 @@ class _CpuRenderer:
         if p == "or_":   # film -> image: restated from main.cpp (no reference entry point)
             f = L.or_film_to_linear; f.restype = None
             f.argtypes = [_dp, C.c_int64, C.c_double, _dp]
+            f = L.or_checkpoint_guard; f.restype = C.c_uint64
+            f.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_int]
+            f = L.or_read_checkpoint; f.restype = C.c_int
+            f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_uint64, C.c_int, _i64p, _dp, _dp, _dp, _i64p]
             f = L.or_film_to_rgb8; f.restype = None
             f.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _u8p, _dp, _dp]
 
@@ -119,6 +123,24 @@ class _CpuRenderer:
 
     def cie_y_integral(self):
         return float(self._f("cie_y_integral")())
+
+    # ---- .ftbuf checkpoints (oracle only)
+    def checkpoint_guard(self, n_tris, n_spheres, n_emitters, total_power, mode, rx, ry):
+        return int(self.lib.or_checkpoint_guard(n_tris, n_spheres, n_emitters, float(total_power),
+                                                ord(mode) if isinstance(mode, str) else int(mode), rx, ry))
+
+    def read_checkpoint(self, path, rx, ry, guard, mode="B"):
+        """readCheckpoint: (status, N, energy[5], xyz (rx*ry, 3), hits, trailing bytes)."""
+        import os as _os
+        n = C.c_int64(0)
+        tr = C.c_int64(0)
+        en = np.zeros(5)
+        xyz = np.zeros((rx * ry, 3))
+        hits = np.zeros(rx * ry)
+        rc = self.lib.or_read_checkpoint(_os.fsencode(str(path)), rx, ry, int(guard),
+                                         ord(mode) if isinstance(mode, str) else int(mode), C.byref(n),
+                                         _ptr(en, _dp), _ptr(xyz, _dp), _ptr(hits, _dp), C.byref(tr))
+        return rc, n.value, en, xyz, hits, tr.value
 
     # ---- film -> image (oracle only)
     def film_to_linear(self, xyz, n_photons):

@@ -18,10 +18,12 @@
 //   :1612-1733 (specular interactions), :2252-2631 (tracePhotonHero + loop),
 //   :2657-2694 (refractOrReflect), main.cpp:11305-11340 (thread pool + seeding),
 //   main.cpp:10265-10348 + color.h:41-53 (filmToLinear, filmToRgb8, xyzToLinearSrgb,
-//   srgbGamma; without the optional denoiser).
+//   srgbGamma; without the optional denoiser), main.cpp:12317-12329, 12467-12483
+//   (checkpointGuard, readCheckpoint).
 #include "gp_oracle.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -1124,6 +1126,60 @@ void or_film_to_rgb8(const double* xyz, int W, int H, double N, double expComp, 
         }
     if (eAutoOut) *eAutoOut = eAuto;
     if (exposureOut) *exposureOut = exposure;
+}
+
+
+// ---- .ftbuf checkpoints (src/main.cpp:12317-12329 checkpointGuard, :12467-12483
+// readCheckpoint). Returns 0 on success, 1 unreadable / not FTBUF01, 2 mismatch of
+// resolution, mode or guard (readCheckpoint's "starting fresh"), 3 truncated. On
+// success fills the header fields, xyz (rx*ry*3) and hits (rx*ry) when non-null, and
+// *trailing = number of bytes after the film blobs (optional sections).
+uint64_t or_checkpoint_guard(int64_t nTris, int64_t nSpheres, int64_t nEmitters, double totalPower,
+                             int mode, int res, int resY) {
+    uint64_t h = 14695981039346656037ULL;
+    auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ULL; };
+    mix((uint64_t)nTris);
+    mix((uint64_t)nSpheres);
+    mix((uint64_t)nEmitters);
+    uint64_t tp;
+    std::memcpy(&tp, &totalPower, sizeof tp);
+    mix(tp);
+    mix((uint64_t)(unsigned char)mode);
+    mix((uint64_t)(unsigned)res);
+    mix((uint64_t)(unsigned)resY);
+    return h;
+}
+
+int or_read_checkpoint(const char* path, int res, int resY, uint64_t guard, int mode, long long* N,
+                       double* energy5, double* xyz, double* hits, long long* trailing) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return 1;
+    auto rd = [&](void* p, size_t n) { return std::fread(p, 1, n, f) == n; };
+    char magic[8];
+    int rc = 0;
+    int32_t rx = 0, ry = 0, m = 0;
+    long long n = 0;
+    double en[5] = {0, 0, 0, 0, 0};
+    uint64_t g = 0;
+    if (!rd(magic, 8) || std::memcmp(magic, "FTBUF01\n", 8) != 0) rc = 1;
+    else if (!rd(&rx, 4) || !rd(&ry, 4) || !rd(&m, 4) || !rd(&n, 8) || !rd(en, sizeof en) || !rd(&g, 8)) rc = 3;
+    else if (rx != res || ry != resY || m != (int32_t)(unsigned char)mode || g != guard) rc = 2;
+    else {
+        const size_t np = (size_t)rx * ry;
+        std::vector<double> x(np * 3), h(np);
+        if (!rd(x.data(), x.size() * sizeof(double)) || !rd(h.data(), h.size() * sizeof(double))) rc = 3;
+        else {
+            if (xyz) std::memcpy(xyz, x.data(), x.size() * sizeof(double));
+            if (hits) std::memcpy(hits, h.data(), h.size() * sizeof(double));
+            if (N) *N = n;
+            if (energy5) std::memcpy(energy5, en, sizeof en);
+            const long pos = std::ftell(f);
+            std::fseek(f, 0, SEEK_END);
+            if (trailing) *trailing = std::ftell(f) - pos;
+        }
+    }
+    std::fclose(f);
+    return rc;
 }
 
 }  // extern "C"

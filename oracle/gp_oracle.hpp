@@ -43,6 +43,10 @@ uint64_t or_mix64(uint64_t x);
 void or_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out);
 void or_cie(const double* lam, int n, double* xyz);
 double or_cie_y_integral();
+uint64_t or_checkpoint_guard(int64_t nTris, int64_t nSpheres, int64_t nEmitters, double totalPower,
+                             int mode, int res, int resY);
+int or_read_checkpoint(const char* path, int res, int resY, uint64_t guard, int mode, long long* N,
+                       double* energy5, double* xyz, double* hits, long long* trailing);
 void or_film_to_linear(const double* xyz, int64_t npix, double N, double* lin);
 void or_film_to_rgb8(const double* xyz, int W, int H, double N, double expComp, int absolute,
                      double* lockAnchor, uint8_t* img, double* eAutoOut, double* exposureOut);
