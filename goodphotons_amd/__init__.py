@@ -43,6 +43,7 @@ class Library:
         L.gp_last_error.restype = C.c_char_p
         L.gp_device_query.argtypes = [C.POINTER(abi.DeviceInfo)]
         L.gp_scene_create.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(vp)]
+        L.gp_scene_create_ex.argtypes = [C.POINTER(abi.SceneDesc), C.c_int32, C.POINTER(vp)]
         L.gp_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
         L.gp_scene_destroy.argtypes = [vp]
         L.gp_scene_destroy.restype = None
@@ -116,12 +117,16 @@ def load() -> Library:
 class GpuScene:
     """A scene resident in HBM (BVH, triangles, materials, emitter tables)."""
 
-    def __init__(self, sd, lib: Library | None = None):
+    BUILDERS = {"host": 0, "device": 1}
+
+    def __init__(self, sd, lib: Library | None = None, builder: str = "host"):
+        """builder: "host" (binned SAH, gp_scene_create) or "device" (linear BVH built
+        on the GPU, gp_scene_create_ex)."""
         self.L = lib or load()
         self.sd = sd
         self._desc = sd.desc()
         h = C.c_void_p()
-        self.L.check(self.L.lib.gp_scene_create(C.byref(self._desc), C.byref(h)))
+        self.L.check(self.L.lib.gp_scene_create_ex(C.byref(self._desc), self.BUILDERS[builder], C.byref(h)))
         self.h = h
 
     def info(self) -> dict:

@@ -1,6 +1,7 @@
 // C ABI of the forward light-tracing core (include/goodphotons.h): scenes,
 // films, gp_trace (one job through the wavefront engine), probes and tuning.
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -8,6 +9,7 @@
 #include <stdexcept>
 
 #include "gp_bvh.hpp"
+#include "gp_bvh_device.hpp"
 #include "gp_runtime.hpp"
 
 namespace gp {
@@ -133,7 +135,7 @@ gp_status gp_device_query(gp_device_info* out) {
 }
 
 // ------------------------------------------------------------------ scenes
-gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
+static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out) {
     if (!d || !out) return fail(GP_ERR_INVALID, "null argument");
     *out = nullptr;
     if (d->n_tris <= 0 || !d->verts || !d->mat_ids) return fail(GP_ERR_INVALID, "scene needs triangles");
@@ -172,26 +174,12 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
     gp_status st = requireDevice();
     if (st != GP_OK) return st;
 
-    auto t0 = std::chrono::steady_clock::now();
-    BvhResult bvh;
-    try {
-        buildBvh(d->verts, d->n_tris, 8, bvh);
-    } catch (const std::exception& ex) {
-        return fail(GP_ERR_INVALID, ex.what());
-    }
-    auto t1 = std::chrono::steady_clock::now();
-
     std::unique_ptr<gp_scene> s(new gp_scene());
     s->nTris = d->n_tris;
-    s->nNodes = (int64_t)(bvh.pairs.size() / 16);
-    s->depth = bvh.depth;
-    s->buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
-
+    s->builder = builder;
     const size_t n = (size_t)d->n_tris;
     std::vector<float4> tris(3 * n), trin(n);
-    s->leafTris.resize(9 * n);
-    for (size_t k = 0; k < n; ++k) {
-        const int prim = bvh.order[k];
+    auto fillTri = [&](size_t k, int prim) {
         const float* v = d->verts + 9 * (size_t)prim;
         const int mat = d->mat_ids[prim];
         tris[3 * k + 0] = make_float4(v[0], v[1], v[2], 0.f);
@@ -199,7 +187,6 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
         tris[3 * k + 2] = make_float4(v[6], v[7], v[8], 0.f);
         std::memcpy(&tris[3 * k + 0].w, &prim, 4);
         std::memcpy(&tris[3 * k + 1].w, &mat, 4);
-        std::memcpy(&s->leafTris[9 * k], v, 9 * sizeof(float));
         double e1[3] = {(double)v[3] - v[0], (double)v[4] - v[1], (double)v[5] - v[2]};
         double e2[3] = {(double)v[6] - v[0], (double)v[7] - v[1], (double)v[8] - v[2]};
         double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
@@ -207,8 +194,54 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
         if (l > 0) { c[0] /= l; c[1] /= l; c[2] /= l; } else { c[0] = 0; c[1] = 0; c[2] = 1; }
         trin[k] = make_float4((float)c[0], (float)c[1], (float)c[2], 0.f);
         std::memcpy(&trin[k].w, &mat, 4);
+    };
+    DeviceBvh dbvh;
+    if (builder == GP_BVH_DEVICE_LBVH) {
+        // input order; the device sorts, builds and gathers
+        float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (size_t k = 0; k < n; ++k) {
+            fillTri(k, (int)k);
+            const float* v = d->verts + 9 * k;
+            for (int a = 0; a < 3; ++a) {
+                const float c = 0.5f * (std::min({v[a], v[3 + a], v[6 + a]}) + std::max({v[a], v[3 + a], v[6 + a]}));
+                clo[a] = std::min(clo[a], c);
+                chi[a] = std::max(chi[a], c);
+            }
+        }
+        DevBuf dIn, dInN;
+        GP_HIP(hipMalloc(&dIn.p, tris.size() * sizeof(float4)));
+        GP_HIP(hipMalloc(&dInN.p, trin.size() * sizeof(float4)));
+        GP_HIP(hipMemcpy(dIn.p, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+        GP_HIP(hipMemcpy(dInN.p, trin.data(), trin.size() * sizeof(float4), hipMemcpyHostToDevice));
+        if ((st = buildBvhDevice((const float4*)dIn.p, (const float4*)dInN.p, d->n_tris, clo, chi, dbvh)) != GP_OK)
+            return st;
+        s->nNodes = dbvh.nPairs;
+        s->depth = dbvh.depth;
+        s->buildMs = dbvh.buildMs;
+        s->dNodes = dbvh.nodes;
+        s->dTris = dbvh.tris;
+        s->dTrin = dbvh.trin;
+        s->deviceBytes += (int64_t)(dbvh.nPairs * 64 + (int64_t)n * 64);
+    } else {
+        auto t0 = std::chrono::steady_clock::now();
+        BvhResult bvh;
+        try {
+            buildBvh(d->verts, d->n_tris, 8, bvh);
+        } catch (const std::exception& ex) {
+            return fail(GP_ERR_INVALID, ex.what());
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        s->nNodes = (int64_t)(bvh.pairs.size() / 16);
+        s->depth = bvh.depth;
+        s->buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        s->leafTris.resize(9 * n);
+        for (size_t k = 0; k < n; ++k) {
+            const int prim = bvh.order[k];
+            fillTri(k, prim);
+            std::memcpy(&s->leafTris[9 * k], d->verts + 9 * (size_t)prim, 9 * sizeof(float));
+        }
+        s->pairs = std::move(bvh.pairs);
     }
-    s->pairs = std::move(bvh.pairs);
 
     std::vector<DevMaterial> mats((size_t)d->n_materials);
     for (int i = 0; i < d->n_materials; ++i) {
@@ -248,9 +281,11 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
         s->deviceBytes += (int64_t)bytes;
         return GP_OK;
     };
-    if ((st = upload(s->pairs.data(), s->pairs.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-    if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-    if ((st = upload(trin.data(), trin.size() * sizeof(float4), (void**)&s->dTrin)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+    if (builder != GP_BVH_DEVICE_LBVH) {
+        if ((st = upload(s->pairs.data(), s->pairs.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+        if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+        if ((st = upload(trin.data(), trin.size() * sizeof(float4), (void**)&s->dTrin)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+    }
     if ((st = upload(mats.data(), mats.size() * sizeof(DevMaterial), (void**)&s->dMats)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
     if ((st = upload(cdfs.data(), cdfs.size() * sizeof(double), (void**)&s->dCdfs)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
     if ((st = upload(emCdf.data(), emCdf.size() * sizeof(double), (void**)&s->dEmCdf)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
@@ -272,6 +307,15 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
     if ((st = upload(ems.data(), ems.size() * sizeof(DevEmitter), (void**)&s->dEms)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
     *out = s.release();
     return GP_OK;
+}
+
+gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
+    return createScene(d, GP_BVH_HOST_SAH, out);
+}
+
+gp_status gp_scene_create_ex(const gp_scene_desc* d, int32_t builder, gp_scene** out) {
+    if (builder != GP_BVH_HOST_SAH && builder != GP_BVH_DEVICE_LBVH) return fail(GP_ERR_INVALID, "unknown BVH builder");
+    return createScene(d, builder, out);
 }
 
 gp_status gp_scene_get_info(const gp_scene* s, gp_scene_info* out) {

@@ -1,6 +1,8 @@
 // Host runtime: device discovery, camera basis, film scaling, wavefront engine.
 #include "gp_runtime.hpp"
 
+#include "gp_bvh_device.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -139,9 +141,21 @@ static double pointBoxDist2(const double* p, const float* lo, const float* hi) {
 
 double sceneMinDistance(const gp_scene& s, const double p[3]) {
     double best = INFINITY;
+    if (s.pairs.empty()) {
+        // tree built on the device: brute force there, once per eye position
+        const std::array<double, 3> key{p[0], p[1], p[2]};
+        auto it = s.eyeDist2.find(key);
+        if (it != s.eyeDist2.end()) {
+            best = it->second;
+        } else {
+            double d2 = 0.0;   // on failure the bound degrades to the 1e-6 floor (smaller scale)
+            if (deviceMinDistance2(s.dTris, s.nTris, p, d2) == GP_OK) s.eyeDist2[key] = d2;
+            best = d2;
+        }
+    }
     // Depth-first over the pair nodes, nearer child first, pruned by box distance.
     std::vector<int> stack;
-    stack.push_back(0);
+    if (!s.pairs.empty()) stack.push_back(0);
     auto visitLeaf = [&](int code) {
         int c = ~code;
         int first = c >> 3, count = (c & 7) + 1;

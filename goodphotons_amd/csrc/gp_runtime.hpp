@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,10 +29,13 @@ struct gp_scene {
     double basePower = 0.0;
     int64_t deviceBytes = 0;
     double buildMs = 0.0;
-    // host copies for camera-distance bounds
+    // host copies for camera-distance bounds (empty when the tree was built on the
+    // device: the bound is then computed there and cached per eye position)
     std::vector<float> pairs;          // 16 floats per pair node
     std::vector<float> leafTris;       // 9 floats per triangle, leaf order
     std::vector<std::array<double, 12>> emitQuads;   // origin, u, v, normal
+    int builder = GP_BVH_HOST_SAH;
+    mutable std::map<std::array<double, 3>, double> eyeDist2;
 };
 
 struct gp_film {

@@ -12,6 +12,8 @@
  *                             (src/render_cuda.h:24, :27, :32)
  *   gp_scene_create        <- buildUpload / buildUploadScene + cudaForwardSupported
  *                             (src/render_cuda.cu:13600, :12692-13548, :12504-12631)
+ *   gp_scene_create_ex     <- the same, with the BVH built on the device instead of
+ *                             Bvh::build (src/bvh.h:89-176) on the host
  *   gp_scene_destroy       <- freeUpload (src/render_cuda.cu:12683)
  *   gp_film_*              <- the device film inside renderForwardCuda
  *                             (src/render_cuda.cu:13722-13746) and the resident
@@ -201,6 +203,13 @@ gp_status gp_device_query(gp_device_info* out);
  * triangles, materials and emitter tables. Fails with GP_ERR_UNSUPPORTED for
  * features outside the set above (the reference's cudaForwardSupported gate). */
 gp_status gp_scene_create(const gp_scene_desc* desc, gp_scene** out);
+/* BVH builders for gp_scene_create_ex. GP_BVH_HOST_SAH is gp_scene_create's: binned
+ * SAH on the host (3 axes x 32 bins), the higher-quality tree. GP_BVH_DEVICE_LBVH
+ * builds a linear BVH on the device (Morton order, Karras 2012) in milliseconds,
+ * for large or per-frame geometry; closest-hit results are the same either way. */
+#define GP_BVH_HOST_SAH 0
+#define GP_BVH_DEVICE_LBVH 1
+gp_status gp_scene_create_ex(const gp_scene_desc* desc, int32_t bvh_builder, gp_scene** out);
 gp_status gp_scene_get_info(const gp_scene* scene, gp_scene_info* out);
 void gp_scene_destroy(gp_scene* scene);
 
