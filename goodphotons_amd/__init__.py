@@ -65,6 +65,8 @@ class Library:
                                       C.POINTER(C.c_uint8), C.POINTER(C.c_double)]
         L.gp_trace.argtypes = [vp, C.POINTER(abi.CameraDesc), vp, C.POINTER(abi.TraceParams),
                                C.POINTER(abi.Energy), C.POINTER(abi.TraceStats)]
+        L.gp_trace_shared.argtypes = [vp, C.c_int32, C.POINTER(abi.CameraDesc), C.POINTER(vp),
+                                      C.POINTER(abi.TraceParams), C.POINTER(abi.Energy), C.POINTER(abi.TraceStats)]
         L.gp_probe_closest_hit.argtypes = [vp, C.POINTER(C.c_float), C.c_int64, C.c_float,
                                            C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.gp_probe_any_hit.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
@@ -317,6 +319,31 @@ def render_forward(scene: GpuScene, cam, n, seed_base=0, hero_c=4, max_depth=32,
     finally:
         film.close()
     return xyz, hits, e.as_dict(), st.as_dict()
+
+
+def trace_shared(scene: GpuScene, cams, films, n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4,
+                 max_depth=32, rr_depth=0, energy: abi.Energy | None = None):
+    """One photon flight splatted into every camera (gp_trace_shared)."""
+    p = trace_params(n, seed_base, salt, hero_c, max_depth, rr_depth)
+    e = energy if energy is not None else abi.Energy()
+    st = abi.TraceStats()
+    ca = (abi.CameraDesc * len(cams))(*cams)
+    fa = (C.c_void_p * len(films))(*[f.h.value for f in films])
+    scene.L.check(scene.L.lib.gp_trace_shared(scene.h, len(cams), ca, fa, C.byref(p), C.byref(e), C.byref(st)))
+    return e, st
+
+
+def render_forward_shared(scene: GpuScene, cams, n, seed_base=0, hero_c=4, max_depth=32, rr_depth=0,
+                          salt=abi.REFERENCE_SALT):
+    """renderForwardSharedCuda: fresh films, one per camera; returns ([(xyz, hits)], energy, stats)."""
+    films = [Film(c.res_x, c.res_y, max_photons=max(1, int(n)), lib=scene.L) for c in cams]
+    try:
+        e, st = trace_shared(scene, cams, films, n, seed_base, salt, hero_c, max_depth, rr_depth)
+        out = [f.download() for f in films]
+    finally:
+        for f in films:
+            f.close()
+    return out, e.as_dict(), st.as_dict()
 
 
 class Server:

@@ -120,7 +120,7 @@ EXPORTED_SYMBOLS = [
     "gp_film_create", "gp_film_reset", "gp_film_download", "gp_film_download_raw",
     "gp_film_destroy", "gp_film_to_linear", "gp_film_tonemap",
     "gp_checkpoint_guard", "gp_film_save_checkpoint", "gp_film_load_checkpoint",
-    "gp_trace", "gp_probe_closest_hit", "gp_probe_any_hit",
+    "gp_trace", "gp_trace_shared", "gp_probe_closest_hit", "gp_probe_any_hit",
     "gp_get_tuning", "gp_set_tuning", "gp_autotune",
     "gp_server_create", "gp_server_submit", "gp_server_step", "gp_server_poll", "gp_server_take_result",
     "gp_server_pending", "gp_server_destroy",

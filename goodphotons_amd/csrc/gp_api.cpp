@@ -399,9 +399,10 @@ void gp_film_destroy(gp_film* f) {
 }
 
 // ------------------------------------------------------------------ trace
-gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const gp_trace_params* p,
-                   gp_energy* energy, gp_trace_stats* stats) {
-    if (!s || !cam || !film || !p) return fail(GP_ERR_INVALID, "null argument");
+gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cams, gp_film* const* films,
+                          const gp_trace_params* p, gp_energy* energy, gp_trace_stats* stats) {
+    if (!s || !cams || !films || !p) return fail(GP_ERR_INVALID, "null argument");
+    if (n_cams < 1 || n_cams > kMaxCams) return fail(GP_ERR_INVALID, "n_cams must be 1..16");
     if (p->n_photons < 0) return fail(GP_ERR_INVALID, "negative photon count");
     if (p->hero_c < 1 || p->hero_c > kMaxHero) return fail(GP_ERR_INVALID, "hero_c must be 1..4");
     if (p->max_depth < 1 || p->max_depth > 250) return fail(GP_ERR_INVALID, "max_depth must be 1..250");
@@ -413,7 +414,7 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     const int lds = E.tun.lds_stack;
     const int needSpill = std::max(0, s->depth + 1 - lds);
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams)) != GP_OK) return st;
     hipStream_t strm = device().stream;
 
     gp_trace_stats acc{};
@@ -429,7 +430,7 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
         cp.n_photons = n;
         cp.seed_base = p->seed_base + (uint64_t)done;
         DevJob J;
-        if ((st = fillJob(s, *cam, film, cp, 0, J)) != GP_OK) return st;
+        if ((st = fillJob(s, cams, films, n_cams, cp, 0, J, E.hJobCams)) != GP_OK) return st;
         E.hJobs[0] = J;
         Ctrl& C = *E.hCtrl;
         E.initCtrl();
@@ -441,6 +442,7 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
         C.active[0] = 1;
         C.lastEmitIter[0] = -1;
         GP_HIP(hipMemcpyAsync(E.dJobs, E.hJobs, sizeof(DevJob), hipMemcpyHostToDevice, strm));
+        GP_HIP(hipMemcpyAsync(E.dJobCams, E.hJobCams, sizeof(DevCamera) * (size_t)n_cams, hipMemcpyHostToDevice, strm));
         GP_HIP(hipMemcpyAsync(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice, strm));
         if ((st = E.resetJobAcc(0)) != GP_OK) return st;
         E.parity = 0;
@@ -487,6 +489,12 @@ gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const 
     }
     if (stats) *stats = acc;
     return GP_OK;
+}
+
+gp_status gp_trace(gp_scene* s, const gp_camera_desc* cam, gp_film* film, const gp_trace_params* p,
+                   gp_energy* energy, gp_trace_stats* stats) {
+    if (!cam || !film) return fail(GP_ERR_INVALID, "null argument");
+    return gp_trace_shared(s, 1, cam, &film, p, energy, stats);
 }
 
 // ------------------------------------------------------------------ probes

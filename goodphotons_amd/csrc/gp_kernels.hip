@@ -136,28 +136,28 @@ __device__ __forceinline__ unsigned segAppend(unsigned (*cnt)[kLine], unsigned c
 // Camera connection record for a diffuse (or emitter) vertex at p with shading
 // normal n (flat triangles: n is also the oriented geometric normal, so the
 // reference's terminator factor and adjoint correction are exactly 1).
-__device__ __forceinline__ bool makeConnection(const DevJob& J, int job, F3 p, F3 n, const float* lam,
-                                               const float* beta, const float* rho, int nUp,
-                                               float4& co, float4& cd, float4& cx) {
-    const F3 eye = load3(J.eye);
+__device__ __forceinline__ bool makeConnection(const DevCamera& K, unsigned tagJobCam, F3 p, F3 n,
+                                               const float* lam, const float* beta, const float* rho,
+                                               int nUp, float4& co, float4& cd, float4& cx) {
+    const F3 eye = load3(K.eye);
     F3 toCam = eye - p;
     float dist = sqrtf(dot(toCam, toCam));
     F3 wdir = toCam * (1.0f / dist);
     float cosSurf = dot(n, wdir);
     if (!(cosSurf > 0.0f)) return false;
     F3 dd = p - eye;
-    float cz = dot(dd, load3(J.w));
+    float cz = dot(dd, load3(K.w));
     if (!(cz > 1e-9f)) return false;
-    float ix = (dot(dd, load3(J.u)) / cz) / J.tanHalfX;
-    float iy = (dot(dd, load3(J.v)) / cz) / J.tanHalfY;
+    float ix = (dot(dd, load3(K.u)) / cz) / K.tanHalfX;
+    float iy = (dot(dd, load3(K.v)) / cz) / K.tanHalfY;
     if (!(ix >= -1.0f && ix < 1.0f && iy >= -1.0f && iy < 1.0f)) return false;
-    int px = (int)((ix * 0.5f + 0.5f) * (float)J.resX);
-    int py = (int)((iy * 0.5f + 0.5f) * (float)J.resY);
-    px = min(max(px, 0), J.resX - 1);
-    py = min(max(py, 0), J.resY - 1);
+    int px = (int)((ix * 0.5f + 0.5f) * (float)K.resX);
+    int py = (int)((iy * 0.5f + 0.5f) * (float)K.resY);
+    px = min(max(px, 0), K.resX - 1);
+    py = min(max(py, 0), K.resY - 1);
     float dist2 = dot(dd, dd);
     float cosCam = cz / sqrtf(dist2);
-    float denom = dist2 * (J.pixelArea * cosCam * cosCam * cosCam);
+    float denom = dist2 * (K.pixelArea * cosCam * cosCam * cosCam);
     float X = 0.f, Y = 0.f, Z = 0.f;
     for (int i = 0; i < nUp; ++i) {
         float c = beta[i] * (rho[i] * kInvPi) * cosSurf / denom;
@@ -168,9 +168,28 @@ __device__ __forceinline__ bool makeConnection(const DevJob& J, int job, F3 p, F
     }
     F3 org = p + n * kRayEps;
     co = make_float4(org.x, org.y, org.z, (dist - 2.0f * kRayEps) - kRayEps);
-    cd = make_float4(wdir.x, wdir.y, wdir.z, __int_as_float(py * J.resX + px));
-    cx = make_float4(X, Y, Z, __int_as_float(job | (nUp << 16)));
+    cd = make_float4(wdir.x, wdir.y, wdir.z, __int_as_float(py * K.resX + px));
+    cx = make_float4(X, Y, Z, __int_as_float(tagJobCam | ((unsigned)nUp << 16)));
     return true;
+}
+
+// Connections of one vertex to every camera of its job, appended to the workgroup's
+// segment of the connection queue. Called by all lanes of the wave (nCams = 0 on
+// lanes without a vertex); the loop runs while any lane has a camera left.
+__device__ __forceinline__ void connectCameras(const Queues& q, Ctrl* c, const DevCamera* cams, int nCams,
+                                               unsigned job, F3 p, F3 n, const float* lam,
+                                               const float* beta, const float* rho, int nUp) {
+    for (int k = 0; __any(k < nCams); ++k) {
+        float4 co, cd, cx;
+        bool conn = false;
+        if (k < nCams) conn = makeConnection(cams[k], connTag(job, 0u, (unsigned)k), p, n, lam, beta, rho, nUp, co, cd, cx);
+        const unsigned ci = segAppend(c->connCount, q.connSegCap, conn, &c->overflow);
+        if (conn && ci != ~0u) {
+            q.co[ci] = co;
+            q.cd[ci] = cd;
+            q.cx[ci] = cx;
+        }
+    }
 }
 
 }  // namespace
@@ -319,18 +338,10 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         accumulate(eacc, j, eTab);
         accumulate(pacc, j, pTab);
         pacc.v[0] += 1;
-        float4 co, cd, cx;
-        bool conn = false;
-        if (!dead) {
-            eacc.v[0] += toFixed((double)b * (double)C, J.energyScale);
+        if (!dead) eacc.v[0] += toFixed((double)b * (double)C, J.energyScale);
+        {
             const float one[kMaxHero] = {1.f, 1.f, 1.f, 1.f};
-            conn = makeConnection(J, j, org, en, lam, beta, one, C, co, cd, cx);
-        }
-        const unsigned ci = segAppend(c->connCount, q.connSegCap, conn, &c->overflow);
-        if (conn && ci != ~0u) {
-            q.co[ci] = co;
-            q.cd[ci] = cd;
-            q.cx[ci] = cx;
+            connectCameras(q, c, J.cams, dead ? 0 : J.nCams, (unsigned)j, org, en, lam, beta, one, C);
         }
         F3 o = org + dir * kRayEps;
         unsigned slot = 0;
@@ -409,8 +420,14 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
         const unsigned i = sm.slot(li);
         const uint4 s4 = q.st[cur][i];
-        bool alive = false, conn = false;
-        float4 co, cd, cx;
+        bool alive = false;
+        // diffuse vertex to connect to the cameras after shading
+        int camN = 0;
+        const DevCamera* cams = nullptr;
+        unsigned cjob = 0;
+        int cUp = 0;
+        F3 cp{0.f, 0.f, 0.f}, cn{0.f, 0.f, 0.f};
+        float cbeta[kMaxHero] = {0.f, 0.f, 0.f, 0.f}, crho[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
         F3 no, nd;
         float lam[kMaxHero], beta[kMaxHero];
         unsigned meta = s4.w;
@@ -494,7 +511,15 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
                         q0 = (k == 0 || c[k] > q0) ? c[k] : q0;
                     }
                     const bool diffuse = (M.type == 0);
-                    if (diffuse) conn = makeConnection(J, j, p, n, lam, beta, c, nUp, co, cd, cx);
+                    if (diffuse) {
+                        camN = J.nCams;
+                        cams = J.cams;
+                        cjob = (unsigned)j;
+                        cUp = nUp;
+                        cp = p;
+                        cn = n;
+                        for (int k = 0; k < kMaxHero; ++k) { cbeta[k] = beta[k]; crho[k] = c[k]; }
+                    }
                     if (roulette) {
                         if (rng.uniform() >= q0) {
                             eacc.v[0] += toFixed(activeSum(), J.energyScale);
@@ -545,12 +570,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
                 meta = packMeta((unsigned)j, (unsigned)(bounce + 1), (unsigned)nUp);
             }
         }
-        const unsigned ci = segAppend(ctl->connCount, q.connSegCap, conn, &ctl->overflow);
-        if (conn && ci != ~0u) {
-            q.co[ci] = co;
-            q.cd[ci] = cd;
-            q.cx[ci] = cx;
-        }
+        connectCameras(q, ctl, cams, camN, cjob, cp, cn, lam, cbeta, crho, cUp);
         const unsigned pi = segAppend(ctl->pathCount[nxt], q.segCap, alive, &ctl->overflow);
         if (alive && pi != ~0u) {
             q.ro[nxt][pi] = make_float4(no.x, no.y, no.z, 0.f);
@@ -564,7 +584,8 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ connect
-// Splat table: a per-workgroup open-addressing table in LDS keyed by (job, pixel).
+// Splat table: a per-workgroup open-addressing table in LDS keyed by (job, camera,
+// pixel).
 // Unoccluded connections add their integer X, Y, Z and hit count into it with LDS
 // 64-bit atomics. When a connection finds no slot within kSplatProbe probes, the
 // workgroup flushes the whole table to the film and starts a fresh one, so every
@@ -613,7 +634,8 @@ __device__ __forceinline__ void flushSplats(const DevJob* jobs, const unsigned l
         if (key == kSplatEmpty) continue;
         const unsigned long long v = sVal[4 * s + w];
         if (v) {
-            unsigned long long* f = (unsigned long long*)jobs[key >> 32].film;
+            const unsigned jc = (unsigned)(key >> 32);
+            unsigned long long* f = (unsigned long long*)jobs[jc / kMaxCams].cams[jc % kMaxCams].film;
             atomicAdd(f + 4 * (size_t)(key & 0xffffffffull) + w, v);
         }
     }
@@ -653,6 +675,7 @@ __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
             const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
             const unsigned tag = (unsigned)__float_as_int(cx.w);
             const int j = (int)(tag & 0xffffu);
+            const int cam = (int)((tag >> 19) & 15u);
             const DevJob& J = q.jobs[j];
             accumulate(cacc, j, cTab);
             cacc.v[0] += 1;
@@ -660,12 +683,13 @@ __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
             const bool occ = traceAny<SD, SPILL>(J.nodes, J.tris, r, kRayEps, co.w, stk);
             if (!occ) {
                 cacc.v[1] += 1;
+                const DevCamera& K = J.cams[cam];
                 const unsigned pix = (unsigned)__float_as_int(cd.w);
-                X = min(toFixed((double)cx.x, J.filmScale), J.filmQmax);
-                Y = min(toFixed((double)cx.y, J.filmScale), J.filmQmax);
-                Z = min(toFixed((double)cx.z, J.filmScale), J.filmQmax);
-                H = (unsigned long long)(tag >> 16);
-                key = ((unsigned long long)j << 32) | pix;
+                X = min(toFixed((double)cx.x, K.filmScale), K.filmQmax);
+                Y = min(toFixed((double)cx.y, K.filmScale), K.filmQmax);
+                Z = min(toFixed((double)cx.z, K.filmScale), K.filmQmax);
+                H = (unsigned long long)((tag >> 16) & 7u);
+                key = ((unsigned long long)(j * kMaxCams + cam) << 32) | pix;
                 const int s = splatSlot(sKey, key, kSplatProbe);
                 if (s >= 0) splatAdd(sVal, s, X, Y, Z, H);
                 else pending = true;
@@ -733,6 +757,7 @@ __global__ void __launch_bounds__(256) k_install_job(Queues q, JobInstall a) {
     const int slot = a.slot;
     if (threadIdx.x == 0) {
         const_cast<DevJob*>(q.jobs)[slot] = a.job;
+        const_cast<DevCamera*>(a.job.cams)[0] = a.cam;
         c->nextPhoton[slot] = a.next;
         c->endPhoton[slot] = a.end;
         c->lastEmitIter[slot] = (a.end > a.next) ? -1 : (int)c->iter;

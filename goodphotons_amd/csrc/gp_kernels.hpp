@@ -52,7 +52,8 @@ void launchProbeAny(const float4* nodes, const float4* tris, const float* rays, 
 // kernel arguments (copied at launch), so the host needs no buffer that outlives the
 // call and no synchronisation.
 struct JobInstall {
-    DevJob job;
+    DevJob job;                     // job.cams points at the slot's camera table
+    DevCamera cam;                  // its single camera
     unsigned long long next, end;   // photon range
     int32_t slot;
     int32_t pad[3];

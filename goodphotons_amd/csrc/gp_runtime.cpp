@@ -228,14 +228,37 @@ gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
     return GP_OK;
 }
 
-gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
-                  const gp_trace_params& p, int slot, DevJob& J) {
-    CamBasis cb;
-    if (!cameraBasis(cam, cb)) return fail(GP_ERR_INVALID, "degenerate camera");
-    if (cam.res_x != film->rx || cam.res_y != film->ry)
-        return fail(GP_ERR_INVALID, "camera resolution differs from the film's");
-    gp_status st = prepareFilm(film, s, cb, p);
-    if (st != GP_OK) return st;
+gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const* films, int nCams,
+                  const gp_trace_params& p, int slot, DevJob& J, DevCamera* camsOut) {
+    if (nCams < 1 || nCams > kMaxCams) return fail(GP_ERR_INVALID, "1..16 cameras per job");
+    gp_status st;
+    for (int c = 0; c < nCams; ++c) {
+        CamBasis cb;
+        if (!cameraBasis(cams[c], cb)) return fail(GP_ERR_INVALID, "degenerate camera");
+        gp_film* film = films[c];
+        if (!film) return fail(GP_ERR_INVALID, "null film");
+        if (cams[c].res_x != film->rx || cams[c].res_y != film->ry)
+            return fail(GP_ERR_INVALID, "camera resolution differs from the film's");
+        for (int d = 0; d < c; ++d)
+            if (films[d] == film) return fail(GP_ERR_INVALID, "two cameras share one film");
+        if ((st = prepareFilm(film, s, cb, p)) != GP_OK) return st;
+        DevCamera& K = camsOut[c];
+        std::memset(&K, 0, sizeof K);
+        for (int i = 0; i < 3; ++i) {
+            K.eye[i] = (float)cb.eye[i];
+            K.u[i] = (float)cb.u[i];
+            K.v[i] = (float)cb.v[i];
+            K.w[i] = (float)cb.w[i];
+        }
+        K.tanHalfX = (float)cb.tanHalfX;
+        K.tanHalfY = (float)cb.tanHalfY;
+        K.pixelArea = (float)cb.pixelArea;
+        K.resX = cb.resX;
+        K.resY = cb.resY;
+        K.film = (long long*)film->d;
+        K.filmScale = film->scale;
+        K.filmQmax = film->qmax;
+    }
     Engine& E = engine();
     std::memset(&J, 0, sizeof J);
     J.nodes = s->dNodes;
@@ -246,20 +269,8 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
     J.emCdf = s->dEmCdf;
     J.nEm = s->nEm;
     J.basePower = (float)s->basePower;
-    for (int i = 0; i < 3; ++i) {
-        J.eye[i] = (float)cb.eye[i];
-        J.u[i] = (float)cb.u[i];
-        J.v[i] = (float)cb.v[i];
-        J.w[i] = (float)cb.w[i];
-    }
-    J.tanHalfX = (float)cb.tanHalfX;
-    J.tanHalfY = (float)cb.tanHalfY;
-    J.pixelArea = (float)cb.pixelArea;
-    J.resX = cb.resX;
-    J.resY = cb.resY;
-    J.film = (long long*)film->d;
-    J.filmScale = film->scale;
-    J.filmQmax = film->qmax;
+    J.cams = E.dJobCams + (size_t)slot * kMaxCams;
+    J.nCams = nCams;
     J.acc = E.dJobAcc + (size_t)slot * kSegs * kAccRow;
     const double ebits = std::floor(60.0 - std::log2(std::max(1.0, (double)p.n_photons)) - 2.0);
     J.energyScale = std::ldexp(1.0, (int)std::max(8.0, ebits)) / std::max(s->basePower, 1e-300);
@@ -299,8 +310,10 @@ void Engine::releaseQueues() {
     if (hCtrl) hipHostFree(hCtrl);
     if (hJobs) hipHostFree(hJobs);
     if (dJobAcc) hipFree(dJobAcc);
+    if (dJobCams) hipFree(dJobCams);
+    if (hJobCams) hipHostFree(hJobCams);
     dCtrl = nullptr; dJobs = nullptr; hCtrl = nullptr; hJobs = nullptr;
-    dJobAcc = nullptr;
+    dJobAcc = nullptr; dJobCams = nullptr; hJobCams = nullptr;
     int2* sp = q.spill;
     q = Queues{};
     q.spill = sp;
@@ -346,7 +359,7 @@ void Engine::initCtrl() {
     std::memset(hCtrl, 0, sizeof(Ctrl));
     hCtrl->capacity = capacity;
     hCtrl->segCap = segCap;
-    hCtrl->connSegCap = 2 * segCap;
+    hCtrl->connSegCap = 2 * segCap * (unsigned)camCap;
     for (int k = 0; k < kMaxJobs; ++k) hCtrl->lastEmitIter[k] = -1;
 }
 
@@ -370,7 +383,7 @@ gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned lo
     return GP_OK;
 }
 
-gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
+gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams) {
     if (!tunSet) { tun = defaultTuning(); tunSet = true; }
     DeviceState& d = device();
     const int cus = d.cus > 0 ? d.cus : 256;
@@ -380,7 +393,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
     ls.blocksShade = std::max(1, cus * tun.grid_waves_per_cu / 4);
     const size_t threads = (size_t)ls.blocksExtend * 256;
     const unsigned sc = segmentBound(cap, (unsigned)ls.blocksShade);
-    const bool needQueues = (cap != capacity) || sc != segCap || !dCtrl;
+    const int camsNeed = std::max(camCap, std::max(1, cams));
+    const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run
     // when every traced scene fits (needSpill == 0). A change of variant re-captures
@@ -408,10 +422,10 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
             if ((st = alloc(pcap * sizeof(uint4), (void**)&q.st[k])) != GP_OK) return st;
         }
         if ((st = alloc(pcap * sizeof(float2), (void**)&q.hit)) != GP_OK) return st;
-        // Connections per iteration: one per emitted photon (direct view of the light)
-        // plus at most one per live path; k_emit and k_shade run the same grid, so a
-        // segment receives at most segCap of each.
-        const size_t ccap = 2 * pcap;
+        // Connections per iteration and camera: one per emitted photon (direct view of
+        // the light) plus at most one per live path; k_emit and k_shade run the same
+        // grid, so a segment receives at most segCap of each per camera.
+        const size_t ccap = 2 * pcap * (size_t)camsNeed;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.co)) != GP_OK) return st;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.cd)) != GP_OK) return st;
         if ((st = alloc(ccap * sizeof(float4), (void**)&q.cx)) != GP_OK) return st;
@@ -420,6 +434,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
         GP_HIP(hipHostMalloc(&hCtrl, sizeof(Ctrl)));
         GP_HIP(hipHostMalloc(&hJobs, sizeof(DevJob) * kMaxJobs));
         GP_HIP(hipMalloc(&dJobAcc, sizeof(unsigned long long) * kMaxJobs * kSegs * kAccRow));
+        GP_HIP(hipMalloc(&dJobCams, sizeof(DevCamera) * kMaxJobs * kMaxCams));
+        GP_HIP(hipHostMalloc(&hJobCams, sizeof(DevCamera) * kMaxJobs * kMaxCams));
         std::memset(hCtrl, 0, sizeof(Ctrl));
         std::memset(hJobs, 0, sizeof(DevJob) * kMaxJobs);
         capacity = cap;
@@ -427,7 +443,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill) {
         q.ctrl = dCtrl;
         q.jobs = dJobs;
         q.segCap = sc;
-        q.connSegCap = 2 * sc;
+        q.connSegCap = 2 * sc * (unsigned)camsNeed;
+        camCap = camsNeed;
     }
     if (needSpillAlloc) {
         const int depthAlloc = std::max(needSpill, spillDepth);

@@ -99,6 +99,9 @@ struct Engine {
     DevJob* dJobs = nullptr;
     DevJob* hJobs = nullptr;            // pinned
     unsigned long long* dJobAcc = nullptr;       // kMaxJobs * kSegs * kAccRow
+    DevCamera* dJobCams = nullptr;               // kMaxJobs * kMaxCams
+    DevCamera* hJobCams = nullptr;               // pinned staging for gp_trace
+    int camCap = 1;                              // cameras per job the connection queue is sized for
     gp_status resetJobAcc(int slot);             // zero a slot's rows (stream-ordered)
     gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]);
     void initCtrl();                            // zero the host control block, set capacities
@@ -116,7 +119,7 @@ struct Engine {
     void resetProfile();
     gp_status collectProfile();
 
-    gp_status ensure(unsigned cap, int lds, int spillDepth);
+    gp_status ensure(unsigned cap, int lds, int spillDepth, int cams = 1);
     void release();
     void releaseQueues();
     void releaseSpill();
@@ -133,8 +136,10 @@ inline int chunkIterations(const gp_tuning& t) {
 }
 unsigned segmentBound(unsigned cap, unsigned blocks);
 
-gp_status fillJob(const gp_scene* s, const gp_camera_desc& cam, gp_film* film,
-                  const gp_trace_params& p, int slot, DevJob& J);
+// Job record for slot `slot` over cameras cams[0..nCams) and their films; the camera
+// records (to be placed at J.cams) are written to camsOut.
+gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const* films, int nCams,
+                  const gp_trace_params& p, int slot, DevJob& J, DevCamera* camsOut);
 gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
                       const gp_trace_params& p);
 

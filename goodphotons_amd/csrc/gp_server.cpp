@@ -184,7 +184,7 @@ gp_status admit(Engine& E, gp_server* srv) {
         srv->waiting.pop_front();
         JobInstall a{};
         gp_status st = streamFilm(J.desc.camera, J.desc.params.n_photons, &J.film);
-        if (st == GP_OK) st = fillJob(J.desc.scene, J.desc.camera, J.film, J.desc.params, slot, a.job);
+        if (st == GP_OK) st = fillJob(J.desc.scene, &J.desc.camera, &J.film, 1, J.desc.params, slot, a.job, &a.cam);
         if (st != GP_OK) {
             J.status = st;
             J.state = JobState::Finished;

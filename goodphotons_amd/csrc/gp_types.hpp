@@ -56,8 +56,22 @@ struct DevEmitter {
     const double* cdf;  // kLambdaBins + 1 entries, normalised
 };
 
-// One job = one (scene, camera, film, photon budget). A single gp_trace is one job;
-// the batched and serving paths run many jobs in the same queues.
+constexpr int kMaxCams = 16;          // cameras one job splats into (shared photon flight)
+
+// Pinhole camera and the film it splats into (Camera::lookAt basis, float).
+struct DevCamera {
+    float eye[3], u[3], v[3], w[3];
+    float tanHalfX, tanHalfY, pixelArea;
+    int32_t resX, resY;
+    int32_t pad;
+    long long* film;             // res*4 int64
+    double filmScale;            // contribution -> integer units
+    unsigned long long filmQmax; // saturation value of one splat
+};
+
+// One job = one (scene, cameras + films, photon budget). A single gp_trace is one job;
+// the batched and serving paths run many jobs in the same queues. Every photon of a
+// job connects to each of its cameras.
 struct DevJob {
     // scene
     const float4* nodes;
@@ -68,14 +82,10 @@ struct DevJob {
     const double* emCdf;     // power CDF over emitters (n_em entries)
     int32_t nEm;
     float basePower;         // photon power: the single emitter's, else the total
-    // camera (Camera::lookAt basis, float)
-    float eye[3], u[3], v[3], w[3];
-    float tanHalfX, tanHalfY, pixelArea;
-    int32_t resX, resY;
-    // film + ledger
-    long long* film;         // res*4 int64
-    double filmScale;        // contribution -> integer units
-    unsigned long long filmQmax;   // saturation value of one splat
+    // cameras
+    const DevCamera* cams;   // nCams entries
+    int32_t nCams;
+    int32_t pad1;
     // kSegs rows of kAccRow u64: [0..4] energy (emitted, absorbed, sensor, escaped,
     // residual, fixed point), [5..8] counters (photons, segments, connections,
     // splats); workgroup b adds into row b % kSegs
@@ -85,6 +95,12 @@ struct DevJob {
     unsigned long long seedBase, salt;
     int32_t heroC, maxDepth, rrDepth, pad0;
 };
+
+// Connection tag (float bits of the record's 4th word): job slot in bits 0..15,
+// live wavelengths in 16..18, camera in 19..22.
+inline constexpr unsigned connTag(unsigned job, unsigned nUp, unsigned cam) {
+    return job | (nUp << 16) | (cam << 19);
+}
 
 // Device-side wavefront control block.
 struct Ctrl {

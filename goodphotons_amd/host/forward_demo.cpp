@@ -98,6 +98,12 @@ int main(int argc, char** argv) {
     gp::Film f4;
     gp::EnergyReport e4;
     if (!b.download(f4, e4)) return 1;
+    // shared flight: the first camera is the same view, the second a different one
+    gp::Camera cam2 = cam;
+    cam2.eye = {1.3, 0.7, 2.2};
+    gp::EnergyReport e5;
+    std::vector<gp::Film> fs = gp::renderForwardSharedGpu(s, {cam, cam2}, {res, res / 2}, {res, res / 2}, N, e5,
+                                                         true, 'B', 0, true, 4);
     gp::EnergyReport e3;
     gp::Film f3 = gp::renderForwardGpu(s, cam, res, res, N, e3, true, 'A');   // unsupported model
     double sum3 = 0;
@@ -106,7 +112,7 @@ int main(int argc, char** argv) {
                 e1.escaped, e1.residual);
     std::printf("unsupported_model_empty %d\n", (int)(sum3 == 0.0 && (int)f3.xyz.size() == res * res));
     if (!save(out + "_single.bin", f1, e1) || !save(out + "_session.bin", f2, e2) ||
-        !save(out + "_resumed.bin", f4, e4))
+        !save(out + "_resumed.bin", f4, e4) || fs.size() != 2 || !save(out + "_shared0.bin", fs[0], e5))
         return 1;
     return 0;
 }

@@ -181,6 +181,62 @@ inline Film renderForwardGpu(const Scene& scene, const Camera& cam, int resX, in
     return film;
 }
 
+// Shared photon flight into several cameras, as renderForwardSharedCuda
+// (src/render_cuda.h, src/render_cuda.cu:13873-13897): each returned film equals the
+// single-camera renderForwardGpu of the same photons. Up to 16 cameras.
+inline std::vector<Film> renderForwardSharedGpu(const Scene& scene, const std::vector<Camera>& cams,
+                                                const std::vector<int>& resX, const std::vector<int>& resY,
+                                                long long N, EnergyReport& eOut, bool diffraction,
+                                                char camMode, unsigned long long seedBase = 0,
+                                                bool wavefront = false, int heroC = 1) {
+    (void)diffraction;
+    (void)wavefront;
+    const size_t nc = cams.size();
+    std::vector<Film> out(nc);
+    for (size_t c = 0; c < nc; ++c) {
+        out[c].resX = resX[c];
+        out[c].resY = resY[c];
+        out[c].alloc();
+    }
+    if (camMode != 'B' || nc == 0 || nc > 16) {
+        std::fprintf(stderr, "[goodphotons] shared render needs 1..16 pinhole ('B') cameras\n");
+        return out;
+    }
+    std::vector<float> v;
+    std::vector<int32_t> m;
+    gp_scene_desc d = detail::describe(scene, v, m);
+    detail::SceneHandle sh;
+    if (gp_scene_create(&d, &sh.s) != GP_OK) { detail::report("scene"); return out; }
+    std::vector<detail::FilmHandle> fh(nc);
+    std::vector<gp_film*> films(nc);
+    std::vector<gp_camera_desc> cd(nc);
+    for (size_t c = 0; c < nc; ++c) {
+        if (gp_film_create(resX[c], resY[c], N > 0 ? N : 1, &fh[c].f) != GP_OK) { detail::report("film"); return out; }
+        films[c] = fh[c].f;
+        cd[c] = detail::camera(cams[c], resX[c], resY[c]);
+    }
+    gp_trace_params p{};
+    p.n_photons = N;
+    p.seed_base = seedBase;
+    p.salt = kReferenceSalt;
+    p.hero_c = heroC < 1 ? 1 : (heroC > 4 ? 4 : heroC);
+    p.max_depth = gMaxDepth;
+    p.rr_depth = gRrDepth;
+    gp_energy e{};
+    if (gp_trace_shared(sh.s, (int32_t)nc, cd.data(), films.data(), &p, &e, nullptr) != GP_OK) {
+        detail::report("shared trace");
+        return out;
+    }
+    for (size_t c = 0; c < nc; ++c)
+        if (!detail::download(films[c], out[c])) { detail::report("download"); out[c].alloc(); }
+    eOut.emitted += e.emitted;
+    eOut.absorbed += e.absorbed;
+    eOut.sensor += e.sensor;
+    eOut.escaped += e.escaped;
+    eOut.residual += e.residual;
+    return out;
+}
+
 // Resident forward session (one camera): the scene and film stay in HBM; every
 // batch traces N more photons into the running totals.
 class ForwardSession {

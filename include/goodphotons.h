@@ -21,6 +21,8 @@
  *   gp_trace               <- renderForwardCuda (src/render_cuda.h:79-82,
  *                             src/render_cuda.cu:13712-13756) and
  *                             sharedForwardGpuBatch (src/render_cuda.h:136-137)
+ *   gp_trace_shared        <- renderForwardSharedCuda / sharedForwardGpuBatch with
+ *                             several cameras (src/render_cuda.h:121-143)
  *   gp_probe_closest_hit   <- Scene::closestHit / Bvh::traverseClosest
  *                             (src/scene.h:2194, src/bvh.h:180-224) exposed as a
  *                             batch probe in the style of -checklattice
@@ -274,6 +276,15 @@ gp_status gp_film_tonemap(gp_film* film, int64_t n_photons, double exposure_comp
 gp_status gp_trace(gp_scene* scene, const gp_camera_desc* cam, gp_film* film,
                    const gp_trace_params* params, gp_energy* energy,
                    gp_trace_stats* stats);
+/* One photon flight splatted into several cameras, as renderForwardShared /
+ * renderForwardSharedCuda (src/main.cpp:11354-11405, src/render_cuda.cu:13873-13897):
+ * every diffuse vertex (and the emission point) connects to each camera; films[c]
+ * (cams[c]'s resolution, one film per camera) receives exactly what a single-camera
+ * gp_trace of the same photons would splat into it, bit for bit. n_cams 1..16. The
+ * ledger and the photon/segment counts are those of the one flight; connections and
+ * splats are summed over the cameras. */
+gp_status gp_trace_shared(gp_scene* scene, int32_t n_cams, const gp_camera_desc* cams, gp_film* const* films,
+                          const gp_trace_params* params, gp_energy* energy, gp_trace_stats* stats);
 
 /* Closest hit for n rays (host arrays). rays: n*6 floats (origin.xyz, dir.xyz).
  * prim_out: global primitive id or -1; t_out: hit distance (inf on miss). */

@@ -47,6 +47,8 @@ def test_cpp_shim(tmp_path, orc):
     x2, h2, e2 = _read(out + "_session.bin")
     assert np.array_equal(x1, x2) and np.array_equal(h1, h2)
     assert np.allclose(e1, e2, rtol=1e-12)
+    x5, h5, e5 = _read(out + "_shared0.bin")     # camera 0 of a two-camera shared flight
+    assert np.array_equal(x1, x5) and np.array_equal(h1, h5)
     x4, h4, e4 = _read(out + "_resumed.bin")     # saved after batch 1, resumed elsewhere
     assert np.array_equal(x1, x4) and np.array_equal(h1, h4)
     assert np.allclose(e1, e4, rtol=1e-12)
