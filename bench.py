@@ -69,6 +69,29 @@ def measured_traffic(kernel, tuning, photons):
     return rec["kernels"][kernel]["hbm_bytes_per_dispatch"]
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "c1_valu.json")
+# wave64 VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def measured_valu(kernel, tuning, photons, avg_ms):
+    """VALU issue of `kernel` against the wave64 issue peak, from the committed
+    rocprofv3 SQ_INSTS_VALU / SQ_THREAD_CYCLES_VALU pass of this workload (same
+    tuning and photon count), divided by this run's live average launch time."""
+    try:
+        with open(VALU_PROFILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if photons != 1 << 26 or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
+        return None
+    k = rec["kernels"][kernel]
+    rate = k["valu_insts_per_dispatch"] / (avg_ms * 1e-3) / 1e9
+    return {"bound": "valu_issue", "achieved": rate, "peak": VALU_PEAK_G, "unit": "G wave-instructions/s",
+            "frac": rate / VALU_PEAK_G, "lane_utilization": k["lane_utilization"],
+            "source": "profiles/c1_valu.json"}
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -207,6 +230,7 @@ def main():
                 "traffic_source": "profiles/c1_traffic.json" if traffic is not None else None,
                 "avg_launch_ms": avg_ms, "launches": kl[dom],
                 "algorithmic_bytes_per_launch": bytes_per_launch,
+                "valu": measured_valu(dom, {k: getattr(tun, k) for k, _ in tun._fields_}, N, avg_ms),
                 "kernel_ms": kms, "pipeline_bytes_per_segment": total_bytes / max(1, pst["segments"]),
                 "pipeline_GBps": total_bytes / (total_ms * 1e-3) / 1e9}
 

@@ -155,3 +155,48 @@ def test_unsupported_material_rejected(gplib):
     with pytest.raises(gp.GpError) as ei:
         gp.GpuScene(sd, gplib)
     assert ei.value.status == 2
+
+
+@pytest.mark.parametrize("hero_c", [1, 2, 3])
+def test_film_matches_oracle_hero_counts(gplib, orc, hero_c):
+    # hero bundles of 1..3 wavelengths (4 is covered above): stratified secondaries,
+    # de-hero at the dielectrics and the max-over-lambda roulette all change with C
+    sd = scenes.caustics_c1(48)
+    n = 1 << 18
+    gs = gp.GpuScene(sd, gplib)
+    gx, gh, ge, gst = gp.render_forward(gs, sd.camera, n, hero_c=hero_c, max_depth=16)
+    cs = orc.scene(sd)
+    cx, ch, ce, cst = orc.render(cs, sd.camera, n, hero_c=hero_c, max_bounce=16, threads=8)
+    m = image_metrics(gx, cx, n, n, res=(48, 48))
+    print(f"[film] c1 hero_c={hero_c}: {m}", flush=True)
+    assert m["rel_rmse_max"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
+    assert abs(ge["emitted"] / ce["emitted"] - 1) < 1e-6
+    assert abs(gst["segments"] / cst["segments"] - 1) < 2e-3
+    assert abs(gh.sum() / ch.sum() - 1) < 2e-3
+
+
+def test_non_square_film_matches_oracle(gplib, orc):
+    sd = scenes.cornell_c0(64).with_camera_res(80, 36)
+    n = 1 << 18
+    gs = gp.GpuScene(sd, gplib)
+    gx, gh, ge, gst = gp.render_forward(gs, sd.camera, n, hero_c=4, max_depth=8, rr_depth=3)
+    cs = orc.scene(sd)
+    cx, ch, ce, cst = orc.render(cs, sd.camera, n, hero_c=4, max_bounce=8, rr_depth=3, threads=8)
+    m = image_metrics(gx, cx, n, n, res=(80, 36))
+    print(f"[film] c0 80x36: {m}", flush=True)
+    assert m["rel_rmse_max"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
+
+
+def test_empty_and_tiny_traces(gplib):
+    sd = scenes.cornell_c0(16)
+    gs = gp.GpuScene(sd, gplib)
+    f = gp.Film(16, 16, max_photons=8, lib=gplib)
+    e, st = gp.trace(gs, sd.camera, f, 0, max_depth=8)
+    assert st.photons == 0 and st.segments == 0 and e.emitted == 0.0
+    assert not f.raw()[0].any()
+    e, st = gp.trace(gs, sd.camera, f, 1, max_depth=8)
+    assert st.photons == 1 and st.segments >= 1 and e.emitted > 0
+    e, st = gp.trace(gs, sd.camera, f, 7, seed_base=1, max_depth=8)
+    assert st.photons == 7
+    with pytest.raises(gp.GpError):           # beyond the film's photon budget
+        gp.trace(gs, sd.camera, f, 1, seed_base=8, max_depth=8)
