@@ -17,8 +17,8 @@
 //
 // The scene types mirror the subset of the reference's Scene the core supports
 // (triangles, Diffuse / Dielectric / Glossy materials with analytic spectra, quad
-// area lights, pinhole camera); INTEGRATION.md shows the adapter from the
-// reference's own Scene/Camera.
+// area lights, pinhole camera); a caller holding the reference's own Scene/Camera
+// copies those fields into these structs before calling ForwardSession.
 #pragma once
 
 #include <cstdint>

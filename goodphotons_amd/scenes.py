@@ -112,6 +112,16 @@ def glossy(spec: abi.Spectrum, roughness: float) -> abi.Material:
     return m
 
 
+def mirror(spec: abi.Spectrum) -> abi.Material:
+    """Perfect mirror (delta reflection) with spectral reflectance `spec`."""
+    m = abi.Material()
+    m.type = abi.MAT_MIRROR
+    m.roughness = 0.0
+    m.reflect = spec
+    m.ior = abi.spectrum(abi.SPEC_CONST, 1.5)
+    return m
+
+
 def quad_emitter(origin, u, v, normal, spd: abi.Spectrum, collimated=False, beam=(1, 0, 0)):
     e = abi.Emitter()
     abi.vec3(e.origin, origin)
@@ -456,6 +466,26 @@ def glossy_box(res: int = 128) -> SceneData:
     m = s.mat_ids.copy()
     m[22:34] = len(mats) - 1
     return SceneData("glossy_box", s.verts, m, mats, list(s.emitters), s.camera,
+                     n_photons=1 << 20, max_depth=8, rr_depth=0, hero_c=4)
+
+
+def mirror_box(res: int = 128) -> SceneData:
+    """Cornell box with a tinted mirror panel hung 1 mm in front of the (red) left wall
+    and the short box turned into a mirror (six rotated planes, two triangles each); a
+    parity case for the flat-mirror camera connection (the configs themselves have
+    no mirrors). The camera sees the right half of the room in the panel."""
+    s = cornell_c0(res)
+    mats = list(s.materials)
+    mats.append(mirror(abi.spectrum(abi.SPEC_STEP, 0.55, 0.92, 520.0)))
+    mi = len(mats) - 1
+    m = s.mat_ids.copy()
+    m[10:22] = mi
+    x = 0.001
+    panel = np.array([[x, 0.08, 0.1, x, 0.08, 0.92, x, 0.82, 0.92],
+                      [x, 0.08, 0.1, x, 0.82, 0.92, x, 0.82, 0.1]], np.float32)
+    v = np.concatenate([s.verts, panel], axis=0)
+    m = np.concatenate([m, np.array([mi, mi], np.int32)])
+    return SceneData("mirror_box", v, m, mats, list(s.emitters), s.camera,
                      n_photons=1 << 20, max_depth=8, rr_depth=0, hero_c=4)
 
 

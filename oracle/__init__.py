@@ -52,6 +52,8 @@ class _CpuRenderer:
         f = getattr(L, p + "scene_destroy"); f.restype = None; f.argtypes = [C.c_void_p]
         f = getattr(L, p + "scene_bvh_nodes"); f.restype = C.c_int64; f.argtypes = [C.c_void_p]
         f = getattr(L, p + "scene_total_power"); f.restype = C.c_double; f.argtypes = [C.c_void_p]
+        f = getattr(L, p + "scene_mirror_planes"); f.restype = C.c_int
+        f.argtypes = [C.c_void_p, _dp, C.c_int]
         f = getattr(L, p + "closest_hit")
         f.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, _i32p, _dp, _i64p, _i64p]
         f = getattr(L, p + "closest_hit_linear")
@@ -230,6 +232,15 @@ class CpuScene:
     @property
     def total_power(self):
         return float(self.r._f("scene_total_power")(self.h))
+
+    def mirror_planes(self):
+        """(n, 10) float64: unit normal, offset d, padded box lo and hi of each flat
+        mirror plane, in the scene's order."""
+        f = self.r._f("scene_mirror_planes")
+        n = f(self.h, None, 0)
+        out = np.zeros((max(n, 1), 10))
+        f(self.h, out.ctypes.data_as(_dp), n)
+        return out[:n]
 
     def closest_hit(self, rays, tmin=1e-6, stats=False):
         rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)

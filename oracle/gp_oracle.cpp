@@ -23,11 +23,13 @@
 #include "gp_oracle.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -502,16 +504,87 @@ struct Light {
     bool collimated = false;
     LambdaCdf cdf;
 };
+// A flat mirror surface: the coplanar mirror triangles of one plane, reduced to the
+// plane (unit normal n with its first dominant component positive, dot(n, x) = d) and
+// the padded box of its member triangles (scene.h:1606-1642).
+struct MirrorPlane {
+    V3 n;
+    double d = 0;
+    V3 lo, hi;
+    double area = 0;
+};
+constexpr int kMaxMirrorPlanes = 64;
+
+// Materials whose reflection is an exact mirror image (scene.h:356-360; HalfMirror is
+// not among the supported types).
+inline bool planarMirror(const Mat& m) {
+    return m.type == GP_MAT_MIRROR || (m.type == GP_MAT_GLOSSY && m.roughness <= 0.0);
+}
+
 struct Scene {
     std::vector<Tri> tris;
     std::vector<Mat> mats;
     std::vector<Light> lights;
     std::vector<double> lightCdf;
+    std::vector<MirrorPlane> mirrors;
     double totalPower = 0;
     Bvh bvh;
 
+    // Group mirror triangles by plane (scene.h:1646-1724): a quantised key of the
+    // canonical normal (1e-6) and offset (1e-5) picks the entry; entries keep their
+    // order of first appearance. Past kMaxMirrorPlanes the largest total areas stay
+    // (the reference's sort is not stable; equal areas keep their order here).
+    void buildMirrorPlanes() {
+        mirrors.clear();
+        std::map<std::array<long long, 4>, int> byKey;
+        for (const Tri& t : tris) {
+            if (t.mat < 0 || t.mat >= (int)mats.size() || !planarMirror(mats[t.mat])) continue;
+            V3 n = t.gn;
+            const double nl = norm(n);
+            if (nl < 1e-12) continue;
+            n = n * (1.0 / nl);
+            const double ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+            const double lead = (ax >= ay && ax >= az) ? n.x : (ay >= az ? n.y : n.z);
+            if (lead < 0.0) n = neg(n);
+            const double d = dot(n, t.v0);
+            const std::array<long long, 4> key{std::llround(n.x * 1e6), std::llround(n.y * 1e6),
+                                               std::llround(n.z * 1e6), std::llround(d * 1e5)};
+            auto it = byKey.find(key);
+            int idx;
+            if (it == byKey.end()) {
+                idx = (int)mirrors.size();
+                byKey.emplace(key, idx);
+                MirrorPlane mp;
+                mp.n = n;
+                mp.d = d;
+                mp.lo = t.v0;
+                mp.hi = t.v0;
+                mirrors.push_back(mp);
+            } else {
+                idx = it->second;
+            }
+            MirrorPlane& mp = mirrors[(size_t)idx];
+            for (const V3& v : {t.v0, t.v1, t.v2}) {
+                mp.lo = V3{std::min(mp.lo.x, v.x), std::min(mp.lo.y, v.y), std::min(mp.lo.z, v.z)};
+                mp.hi = V3{std::max(mp.hi.x, v.x), std::max(mp.hi.y, v.y), std::max(mp.hi.z, v.z)};
+            }
+            mp.area += 0.5 * norm(cross(t.v1 - t.v0, t.v2 - t.v0));
+        }
+        if ((int)mirrors.size() > kMaxMirrorPlanes) {
+            std::stable_sort(mirrors.begin(), mirrors.end(),
+                             [](const MirrorPlane& a, const MirrorPlane& b) { return a.area > b.area; });
+            mirrors.resize(kMaxMirrorPlanes);
+        }
+        for (MirrorPlane& mp : mirrors) {
+            const double pad = 1e-6 * (1.0 + norm(mp.hi - mp.lo));
+            mp.lo = mp.lo - V3{pad, pad, pad};
+            mp.hi = mp.hi + V3{pad, pad, pad};
+        }
+    }
+
     void build() {
         for (auto& t : tris) t.gn = unit(cross(t.v1 - t.v0, t.v2 - t.v0));
+        buildMirrorPlanes();
         std::vector<Box> boxes;
         boxes.reserve(tris.size());
         const double pad = 1e-6;
@@ -680,6 +753,64 @@ struct Tracer {
         }
     }
 
+    // Pinhole connection of vertex p (normal np, per-wavelength weight w) by
+    // reflection in one flat mirror (render.h:1315-1375): the eye's mirror image turns
+    // p -> R -> eye into the straight segment p -> eye', R is where it crosses the
+    // plane, and the specular geometry factor is 1 / D^2 with D = |p - eye'|. The
+    // light-side leg must be clear, and a closest hit from the eye must land on a
+    // planar mirror at R (within 1e-4 (1 + dE)); the mirror's reflectance is read off
+    // the material that hit found.
+    void mirrorPlane(const MirrorPlane& mp, const V3& p, const V3& np, const double* lam,
+                     const double* beta, const double* w, int nUp) {
+        const V3 eye = cam.eye;
+        const double se = dot(mp.n, eye) - mp.d;
+        const double sp = dot(mp.n, p) - mp.d;
+        if (!(se * sp > 0.0)) return;
+        const V3 eyeM = eye - mp.n * (2.0 * se);
+        V3 wP = eyeM - p;
+        const double D = norm(wP);
+        if (D < 1e-9) return;
+        wP = wP * (1.0 / D);
+        const double cosSurf = dot(np, wP);
+        const double shape = cosSurf <= 0.0 ? -1.0 : cosSurf / kPi;
+        if (shape < 0.0) return;
+        const double denom = dot(mp.n, wP);
+        if (std::fabs(denom) < 1e-12) return;
+        const double dP = (mp.d - dot(mp.n, p)) / denom;
+        if (dP <= 1e-7 || dP >= D - 1e-7) return;
+        const V3 R = p + wP * dP;
+        if (R.x < mp.lo.x || R.x > mp.hi.x || R.y < mp.lo.y || R.y > mp.hi.y || R.z < mp.lo.z ||
+            R.z > mp.hi.z)
+            return;
+        int px, py;
+        double cosCam, dist2e;
+        if (!cam.project(R, px, py, cosCam, dist2e)) return;
+        const double omega = cam.pixelOmega(cosCam);
+        if (omega <= 0.0) return;
+        const double dE = D - dP;
+        const V3 wRE = (eye - R) * (1.0 / dE);
+        ++k.connections;
+        if (S.occluded(p + wP * 1e-6, wP, dP - 2e-6)) return;
+        ++k.connections;
+        const Hit hm = S.closestHit(eye, neg(wRE));
+        if (!hm.valid) return;
+        if (std::fabs(hm.t - dE) > 1e-4 * (1.0 + dE)) return;
+        const Mat& mm = S.mats[hm.mat];
+        if (!planarMirror(mm)) return;
+        ++k.splats;
+        const double invD2Omega = 1.0 / (D * D * omega);
+        for (int i = 0; i < nUp; ++i) {
+            const double refl = clampUnit(specEval(mm.reflect, lam[i]));
+            const double contrib = beta[i] * (w[i] * shape) * refl * invD2Omega;
+            if (contrib <= 0.0) continue;
+            film.add(px, py, V3{xbar(lam[i]), ybar(lam[i]), zbar(lam[i])} * contrib);
+        }
+    }
+    void splatMirrors(const V3& p, const V3& np, const double* lam, const double* beta,
+                      const double* w, int nUp) {
+        for (const MirrorPlane& mp : S.mirrors) mirrorPlane(mp, p, np, lam, beta, w, nUp);
+    }
+
     // Dielectric interface: Fresnel-weighted reflect or Snell refract (exterior
     // index 1), optionally frosted by a glossy lobe.
     void dielectric(const Mat& m, const Hit& h, V3& ro, V3& rd, double lambda, Pcg& rng) {
@@ -741,6 +872,7 @@ struct Tracer {
             double one[8];
             for (int i = 0; i < C; ++i) one[i] = 1.0;
             splatHero(origin, emitN, emitN, emitN, lam, beta, one, C);
+            splatMirrors(origin, emitN, lam, beta, one, C);
         }
         V3 ro = origin + dir * 1e-6, rd = dir;
         for (int bounce = 0; bounce < P.maxBounce; ++bounce) {
@@ -791,6 +923,7 @@ struct Tracer {
             V3 ngo = facingGeoN(h);
             V3 wi = neg(rd);
             splatHero(h.p, h.n, ngo, wi, lam, beta, rho, nUp);
+            splatMirrors(h.p, h.n, lam, beta, rho, nUp);
             const double q = maxLive(rho, nUp);
             if (roulette) {
                 if (rng.uniform() >= q) { e.absorbed += activeSum(); return; }
@@ -858,6 +991,15 @@ void* or_scene_create(const gp_scene_desc* d) { return makeScene(d); }
 void or_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
 int64_t or_scene_bvh_nodes(void* s) { return (int64_t) static_cast<Scene*>(s)->bvh.nodes.size(); }
 double or_scene_total_power(void* s) { return static_cast<Scene*>(s)->totalPower; }
+int or_scene_mirror_planes(void* s, double* out10, int cap) {
+    const auto& m = static_cast<Scene*>(s)->mirrors;
+    for (int i = 0; i < (int)m.size() && i < cap; ++i) {
+        const MirrorPlane& p = m[(size_t)i];
+        const double v[10] = {p.n.x, p.n.y, p.n.z, p.d, p.lo.x, p.lo.y, p.lo.z, p.hi.x, p.hi.y, p.hi.z};
+        std::memcpy(out10 + 10 * i, v, sizeof v);
+    }
+    return (int)m.size();
+}
 
 int or_render(void* sp, const gp_camera_desc* cd, int64_t N, uint64_t seedBase, uint64_t salt,
               int heroC, int maxBounce, int rrDepth, int nThreads, double* xyz, double* hits,

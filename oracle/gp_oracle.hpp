@@ -24,6 +24,9 @@ void* or_scene_create(const gp_scene_desc* d);
 void or_scene_destroy(void* s);
 int64_t or_scene_bvh_nodes(void* s);
 double or_scene_total_power(void* s);
+// Flat mirror planes of the scene (n.xyz, d, lo.xyz, hi.xyz per plane, up to cap);
+// returns their number.
+int or_scene_mirror_planes(void* s, double* out10, int cap);
 
 int or_render(void* s, const gp_camera_desc* cam, int64_t N, uint64_t seedBase, uint64_t salt,
               int heroC, int maxBounce, int rrDepth, int nThreads, double* xyz, double* hits,

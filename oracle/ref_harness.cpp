@@ -114,6 +114,16 @@ void ref_scene_destroy(void* p) { delete static_cast<RefScene*>(p); }
 
 int64_t ref_scene_bvh_nodes(void* p) { return (int64_t) static_cast<RefScene*>(p)->scene.bvh.nodes.size(); }
 double ref_scene_total_power(void* p) { return static_cast<RefScene*>(p)->scene.totalPower; }
+// Scene::mirrorPlanes (src/scene.h:1617-1624) as n.xyz, d, lo.xyz, hi.xyz per plane.
+int ref_scene_mirror_planes(void* p, double* out10, int cap) {
+    const auto& m = static_cast<RefScene*>(p)->scene.mirrorPlanes;
+    for (int i = 0; i < (int)m.size() && i < cap; ++i) {
+        const auto& q = m[(size_t)i];
+        const double v[10] = {q.n.x, q.n.y, q.n.z, q.d, q.lo.x, q.lo.y, q.lo.z, q.hi.x, q.hi.y, q.hi.z};
+        std::memcpy(out10 + 10 * i, v, sizeof v);
+    }
+    return (int)m.size();
+}
 
 // The CPU branch of renderForward (src/main.cpp:11305-11340): T threads, static
 // contiguous slices, per-thread films merged in thread order, photon i seeded by

@@ -108,7 +108,7 @@ def main():
 
     # --- films and energy ledgers (32x32, one thread so accumulation order is fixed)
     cases = [("c0", scenes.cornell_c0(32), 4096), ("c1", scenes.caustics_c1(32), 4096),
-             ("glossy", scenes.glossy_box(32), 4096)]
+             ("glossy", scenes.glossy_box(32), 4096), ("mirror", scenes.mirror_box(32), 4096)]
     for name, sd, n in cases:
         s = R.scene(sd)
         for hc in (1, 4):
@@ -118,6 +118,9 @@ def main():
                                 energy=np.array([e[k] for k in ("emitted", "absorbed", "sensor", "escaped", "residual")]),
                                 ray_queries=np.array([rq["ray_queries"]]), n=np.array([n]),
                                 seed_base=np.array([12345]), max_bounce=np.array([sd.max_depth]))
+    # --- flat mirror planes (Scene::mirrorPlanes) of the mirror scene
+    sd = scenes.mirror_box(32)
+    np.savez_compressed(os.path.join(HERE, "mirror_planes.npz"), planes=R.scene(sd).mirror_planes())
     print("wrote fixtures to", HERE)
 
 

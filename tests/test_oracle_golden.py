@@ -139,7 +139,7 @@ def test_axis_aligned_wall_closed_form(orc):
 
 
 @pytest.mark.parametrize("name,builder", [("c0", scenes.cornell_c0), ("c1", scenes.caustics_c1),
-                                          ("glossy", scenes.glossy_box)])
+                                          ("glossy", scenes.glossy_box), ("mirror", scenes.mirror_box)])
 @pytest.mark.parametrize("hero", [1, 4])
 def test_film_golden(orc, name, builder, hero):
     g = golden(f"film_{name}_h{hero}.npz")
@@ -177,3 +177,44 @@ def test_rr_depth_is_unbiased(orc):
     a = orc.render(s, sd.camera, n, hero_c=4, max_bounce=8, rr_depth=0, threads=8)[0].sum(0)
     b = orc.render(s, sd.camera, n, hero_c=4, max_bounce=8, rr_depth=3, seed_base=n, threads=8)[0].sum(0)
     assert np.all(np.abs(b / a - 1) < 0.01)
+
+
+def test_mirror_planes_golden(orc):
+    # Coplanar mirror triangles collapse to one plane each (the short box's six faces
+    # and the wall panel), with the reference's canonical normal, offset and padded box.
+    g = golden("mirror_planes.npz")["planes"]
+    p = orc.scene(scenes.mirror_box(32)).mirror_planes()
+    assert len(g) == 7 and np.array_equal(p, g)
+
+
+def test_mirror_plane_geometry_closed_form(orc):
+    # Closed forms: a unit normal with its dominant component positive, every member
+    # vertex on the plane (|n.v - d| at float rounding), and the wall panel at x = 0.001.
+    sd = scenes.mirror_box(32)
+    p = orc.scene(sd).mirror_planes()
+    n, d = p[:, :3], p[:, 3]
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-15)
+    lead = n[np.arange(len(n)), np.argmax(np.abs(n), axis=1)]
+    assert np.all(lead > 0)
+    v = sd.verts.reshape(-1, 3, 3).astype(np.float64)[sd.mat_ids == 4]
+    for tri in v:
+        k = np.argmin([np.abs(tri @ nn - dd).max() for nn, dd in zip(n, d)])
+        assert np.abs(tri @ n[k] - d[k]).max() < 1e-6
+        assert np.all(tri >= p[k, 4:7]) and np.all(tri <= p[k, 7:10])
+    wall = p[np.argmax(np.abs(n[:, 0]) * (np.abs(d - 0.001) < 1e-6))]
+    assert np.allclose(wall[:3], [1, 0, 0]) and abs(wall[3] - 0.001) < 1e-9
+
+
+def test_mirror_connection_adds_light(orc):
+    # The mirror panel shows the lit room: the same scene with the mirror material
+    # replaced by a black diffuse one has strictly less light, and the difference
+    # lands in the panel's columns of the image (the left wall).
+    sd = scenes.mirror_box(32)
+    a = orc.render(orc.scene(sd), sd.camera, 40000, hero_c=4, max_bounce=8, threads=8)[0]
+    mats = list(sd.materials)
+    mats[4] = scenes.diffuse(abi.spectrum(abi.SPEC_CONST, 0.0))
+    sd.materials = mats
+    b = orc.render(orc.scene(sd), sd.camera, 40000, hero_c=4, max_bounce=8, threads=8)[0]
+    ya, yb = a[:, 1].reshape(32, 32), b[:, 1].reshape(32, 32)
+    assert ya.sum() > 1.1 * yb.sum()
+    assert ya[:, :8].sum() > 2.0 * yb[:, :8].sum()
