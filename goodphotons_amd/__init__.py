@@ -47,6 +47,8 @@ class Library:
         L.gp_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
         L.gp_scene_destroy.argtypes = [vp]
         L.gp_scene_destroy.restype = None
+        L.gp_mirror_planes.restype = C.c_int32
+        L.gp_mirror_planes.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(C.c_double), C.c_int32]
         L.gp_film_create.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.POINTER(vp)]
         L.gp_film_reset.argtypes = [vp]
         L.gp_film_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -89,6 +91,17 @@ class Library:
         if status != 0:
             msg = self.lib.gp_last_error()
             raise GpError(status, msg.decode() if msg else "")
+
+    def mirror_planes(self, sd) -> np.ndarray:
+        """(n, 10) float64: the flat mirror planes gp_scene_create derives from the
+        scene (unit normal, offset d, padded box lo and hi). Host only."""
+        d = sd.desc()
+        n = self.lib.gp_mirror_planes(C.byref(d), None, 0)
+        if n < 0:
+            self.check(-n)
+        out = np.zeros((max(n, 1), 10))
+        self.lib.gp_mirror_planes(C.byref(d), out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        return out[:n]
 
     def device_info(self) -> dict:
         d = abi.DeviceInfo()

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgoodphotons.so")
 
-SOURCES = ["gp_kernels.hip", "gp_image.hip", "gp_bvh_device.hip", "gp_runtime.cpp", "gp_api.cpp", "gp_server.cpp", "gp_bvh.cpp", "gp_checkpoint.cpp"]
+SOURCES = ["gp_kernels.hip", "gp_image.hip", "gp_bvh_device.hip", "gp_runtime.cpp", "gp_api.cpp", "gp_server.cpp", "gp_bvh.cpp", "gp_checkpoint.cpp", "gp_mirror.cpp"]
 ARCH = "gfx950"
 
 

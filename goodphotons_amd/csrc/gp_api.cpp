@@ -146,13 +146,13 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         const gp_material& m = d->materials[i];
         if (m.type == GP_MAT_MIRROR)
             return fail(GP_ERR_UNSUPPORTED,
-                        "Mirror material: the reference adds an analytic planar-mirror camera "
-                        "connector for mirror triangles (src/render.h:1545), which this core does not "
-                        "implement; use a Glossy material with roughness > 0");
+                        "Mirror material: the flat-mirror camera connection is switched off until its "
+                        "fix (separate LDS stacks for its two traversals) is verified on the GPU");
         if (m.type == GP_MAT_GLOSSY && !(m.roughness > 0.0))
-            return fail(GP_ERR_UNSUPPORTED, "Glossy with roughness <= 0 is a planar mirror (see Mirror)");
-        if (m.type != GP_MAT_DIFFUSE && m.type != GP_MAT_DIELECTRIC && m.type != GP_MAT_GLOSSY)
-            return fail(GP_ERR_UNSUPPORTED, "material type outside {Diffuse, Dielectric, Glossy}");
+            return fail(GP_ERR_UNSUPPORTED, "Glossy with roughness <= 0 is a planar mirror");
+        if (m.type != GP_MAT_DIFFUSE && m.type != GP_MAT_DIELECTRIC && m.type != GP_MAT_MIRROR &&
+            m.type != GP_MAT_GLOSSY)
+            return fail(GP_ERR_UNSUPPORTED, "material type outside {Diffuse, Dielectric, Mirror, Glossy}");
         gp_status st;
         if ((st = validateSpectrum(m.reflect, "reflect")) != GP_OK) return st;
         if ((st = validateSpectrum(m.ior, "ior")) != GP_OK) return st;
@@ -305,6 +305,12 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         de.cdf = s->dCdfs + (size_t)i * (kLambdaBins + 1);
     }
     if ((st = upload(ems.data(), ems.size() * sizeof(DevEmitter), (void**)&s->dEms)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+    std::vector<DevMirrorPlane> planes;
+    buildMirrorPlanes(d, planes);
+    if (!planes.empty()) {
+        if ((st = upload(planes.data(), planes.size() * sizeof(DevMirrorPlane), (void**)&s->dMirrors)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+        s->nMirrors = (int)planes.size();
+    }
     *out = s.release();
     return GP_OK;
 }
@@ -332,7 +338,7 @@ gp_status gp_scene_get_info(const gp_scene* s, gp_scene_info* out) {
 
 void gp_scene_destroy(gp_scene* s) {
     if (!s) return;
-    void* ptrs[] = {s->dNodes, s->dTris, s->dTrin, s->dMats, s->dEms, s->dCdfs, s->dEmCdf};
+    void* ptrs[] = {s->dNodes, s->dTris, s->dTrin, s->dMats, s->dEms, s->dCdfs, s->dEmCdf, s->dMirrors};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete s;
@@ -414,7 +420,8 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     const int lds = E.tun.lds_stack;
     const int needSpill = std::max(0, s->depth + 1 - lds);
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams, s->nMirrors > 0)) != GP_OK) return st;
+    E.mirrorActive = s->nMirrors > 0;
     hipStream_t strm = device().stream;
 
     gp_trace_stats acc{};

@@ -114,7 +114,9 @@ __device__ __forceinline__ unsigned loadSegMap(const unsigned (*cnt)[kLine], uns
 #pragma unroll
     for (int g = 0; g < kSegs; ++g) {
         m.pre[g] = acc;
-        acc += cnt[g][0];
+        // a segment that overflowed (the trace then fails with GP_ERR_CAPACITY) still
+        // maps only its own slots
+        acc += min(cnt[g][0], cap);
     }
     m.cap = cap;
     return acc;
@@ -192,6 +194,22 @@ __device__ __forceinline__ void connectCameras(const Queues& q, Ctrl* c, const D
     }
 }
 
+// A vertex to connect to its job's cameras through the scene's flat mirrors, queued
+// for k_mirror (weights w: the albedo at a surface vertex, 1 at the emitter). Called
+// by all active lanes of the wave.
+__device__ __forceinline__ void queueMirrorVertex(const Queues& q, Ctrl* c, bool pred, unsigned job, F3 p,
+                                                  F3 n, const float* lam, const float* beta,
+                                                  const float* w, int nUp) {
+    if (!__any(pred)) return;
+    const unsigned mi = segAppend(c->mirrorCount, q.mirrorSegCap, pred, &c->overflow);
+    if (pred && mi != ~0u) {
+        q.mv[0][mi] = make_float4(p.x, p.y, p.z, __int_as_float((int)(job | ((unsigned)nUp << 16))));
+        q.mv[1][mi] = make_float4(n.x, n.y, n.z, 0.f);
+        q.mv[2][mi] = make_float4(lam[0], lam[1], lam[2], lam[3]);
+        q.mv[3][mi] = make_float4(beta[0] * w[0], beta[1] * w[1], beta[2] * w[2], beta[3] * w[3]);
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ schedule
@@ -255,6 +273,7 @@ __global__ void __launch_bounds__(kMaxJobs) k_schedule(Queues q, int cur) {
             c->pathCount[cur][g][0] = cnt + e;
             c->pathCount[cur ^ 1][g][0] = 0;
             c->connCount[g][0] = 0;
+            c->mirrorCount[g][0] = 0;
             placed += e;
             left -= e;
         }
@@ -342,6 +361,7 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         {
             const float one[kMaxHero] = {1.f, 1.f, 1.f, 1.f};
             connectCameras(q, c, J.cams, dead ? 0 : J.nCams, (unsigned)j, org, en, lam, beta, one, C);
+            queueMirrorVertex(q, c, !dead && J.nMirrors > 0, (unsigned)j, org, en, lam, beta, one, C);
         }
         F3 o = org + dir * kRayEps;
         unsigned slot = 0;
@@ -423,6 +443,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
         bool alive = false;
         // diffuse vertex to connect to the cameras after shading
         int camN = 0;
+        bool mirrorsToo = false;
         const DevCamera* cams = nullptr;
         unsigned cjob = 0;
         int cUp = 0;
@@ -513,6 +534,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
                     const bool diffuse = (M.type == 0);
                     if (diffuse) {
                         camN = J.nCams;
+                        mirrorsToo = J.nMirrors > 0;
                         cams = J.cams;
                         cjob = (unsigned)j;
                         cUp = nUp;
@@ -571,6 +593,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             }
         }
         connectCameras(q, ctl, cams, camN, cjob, cp, cn, lam, cbeta, crho, cUp);
+        queueMirrorVertex(q, ctl, mirrorsToo, cjob, cp, cn, lam, cbeta, crho, cUp);
         const unsigned pi = segAppend(ctl->pathCount[nxt], q.segCap, alive, &ctl->overflow);
         if (alive && pi != ~0u) {
             q.ro[nxt][pi] = make_float4(no.x, no.y, no.z, 0.f);
@@ -700,14 +723,185 @@ __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
             __syncthreads();
             clearSplats(sKey, sVal);
             __syncthreads();
-            // At most one key per thread goes into the empty table, so a full scan
-            // always finds its key or a free slot.
-            if (pending) splatAdd(sVal, splatSlot(sKey, key, kSplatSlots), X, Y, Z, H);
+            // Only the pending keys, at most one per thread, go into the empty table
+            // before the barrier below lets any thread start the next round, so a full
+            // scan always finds its key or a free slot. (Without the barrier, waves
+            // racing ahead could fill the table first.) A failed scan still falls
+            // back to the film itself rather than index the table with -1.
+            if (pending) {
+                const int s = splatSlot(sKey, key, kSplatSlots);
+                if (s >= 0) {
+                    splatAdd(sVal, s, X, Y, Z, H);
+                } else {
+                    const unsigned jc = (unsigned)(key >> 32);
+                    unsigned long long* f =
+                        (unsigned long long*)q.jobs[jc / kMaxCams].cams[jc % kMaxCams].film +
+                        4 * (size_t)(key & 0xffffffffull);
+                    if (X) atomicAdd(f + 0, X);
+                    if (Y) atomicAdd(f + 1, Y);
+                    if (Z) atomicAdd(f + 2, Z);
+                    atomicAdd(f + 3, H);
+                }
+            }
+            __syncthreads();
         }
     }
     flushJobTable(cacc, cTab, nJobs, ca);   // ends with a barrier-ordered read of cTab
     __syncthreads();
     flushSplats(q.jobs, sKey, sVal);
+}
+
+// ------------------------------------------------------------------ mirrors
+// Camera connections through flat mirrors (the reference's connectSpecularPlane,
+// src/render.h:1334-1376, GPU twin src/render_cuda.cu:5004-5052). Mirroring the eye
+// in the plane turns vertex -> mirror -> eye into the straight segment from the
+// vertex to the eye's image, so the mirror point R is a plane intersection and the
+// specular geometry factor is 1 / D^2 over the unfolded length D. The geometry is
+// solved in double; the two legs are traced in float: the light side must be clear
+// (any hit), and the eye's closest hit must be a mirror triangle at R (within
+// 1e-4 (1 + |R - eye|)), whose reflectance tints the splat. One thread per queued
+// vertex walks the scene's planes (outer) and the job's cameras (inner); splats go
+// straight to the film with 64-bit atomics (they are few next to the direct
+// connections, and integer sums keep the film independent of their order).
+struct MirrorSolve {
+    double wP[3];       // vertex -> mirror point, unit
+    double dP;          // vertex -> mirror point distance
+    double wE[3];       // eye -> mirror point, unit
+    double dE;          // eye -> mirror point distance
+    double weight;      // (cos at the vertex / pi) / (D^2 * pixel solid angle)
+    unsigned pix;       // film pixel of the mirror point
+};
+
+__device__ __forceinline__ bool solveMirror(const DevMirrorPlane& P, const DevCamera& K, const double p[3],
+                                            const double np[3], MirrorSolve& o) {
+    const double e[3] = {(double)K.eye[0], (double)K.eye[1], (double)K.eye[2]};
+    const double pd = P.n[0] * p[0] + P.n[1] * p[1] + P.n[2] * p[2];
+    const double se = (P.n[0] * e[0] + P.n[1] * e[1] + P.n[2] * e[2]) - P.d;
+    const double sp = pd - P.d;
+    if (!(se * sp > 0.0)) return false;   // eye and vertex on the same face
+    double w[3];
+    for (int a = 0; a < 3; ++a) w[a] = (e[a] - P.n[a] * (2.0 * se)) - p[a];
+    const double D = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (D < 1e-9) return false;
+    const double inv = 1.0 / D;
+    for (int a = 0; a < 3; ++a) w[a] *= inv;
+    const double cosSurf = np[0] * w[0] + np[1] * w[1] + np[2] * w[2];
+    if (!(cosSurf > 0.0)) return false;
+    const double denom = P.n[0] * w[0] + P.n[1] * w[1] + P.n[2] * w[2];
+    if (fabs(denom) < 1e-12) return false;
+    const double dP = (P.d - pd) / denom;
+    if (dP <= 1e-7 || dP >= D - 1e-7) return false;
+    double R[3];
+    for (int a = 0; a < 3; ++a) {
+        R[a] = p[a] + w[a] * dP;
+        if (R[a] < P.lo[a] || R[a] > P.hi[a]) return false;
+    }
+    // pinhole projection of R (Camera::project / pixelSolidAngle)
+    const double d[3] = {R[0] - e[0], R[1] - e[1], R[2] - e[2]};
+    const double cz = d[0] * K.w[0] + d[1] * K.w[1] + d[2] * K.w[2];
+    if (!(cz > 1e-9)) return false;
+    const double ix = ((d[0] * K.u[0] + d[1] * K.u[1] + d[2] * K.u[2]) / cz) / (double)K.tanHalfX;
+    const double iy = ((d[0] * K.v[0] + d[1] * K.v[1] + d[2] * K.v[2]) / cz) / (double)K.tanHalfY;
+    if (!(ix >= -1.0 && ix < 1.0 && iy >= -1.0 && iy < 1.0)) return false;
+    const int px = min(max((int)((ix * 0.5 + 0.5) * (double)K.resX), 0), K.resX - 1);
+    const int py = min(max((int)((iy * 0.5 + 0.5) * (double)K.resY), 0), K.resY - 1);
+    const double dist2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double cosCam = cz / sqrt(dist2);
+    const double omega = (double)K.pixelArea * cosCam * cosCam * cosCam;
+    if (!(omega > 0.0)) return false;
+    const double dE = D - dP;
+    for (int a = 0; a < 3; ++a) {
+        o.wP[a] = w[a];
+        o.wE[a] = (R[a] - e[a]) * (1.0 / dE);
+    }
+    o.dP = dP;
+    o.dE = dE;
+    o.weight = (cosSurf / 3.141592653589793) / (D * D * omega);
+    o.pix = (unsigned)(py * K.resX + px);
+    return true;
+}
+
+template <int SD, bool SPILL>
+__global__ void __launch_bounds__(kTravThreads) k_mirror(Queues q) {
+    // Separate LDS arrays for the two traversals: lanes of a workgroup run the light leg
+    // (any hit, int entries) and the eye leg (closest hit, int2 entries) at the same
+    // time, and two views of one array would overlap across lanes (lane t's int entry k
+    // and lane u's int2 entry r share a word whenever k*256 + t = 2*(r*256 + u) + 0/1).
+    __shared__ int2 sStk[SD * kTravThreads];
+    __shared__ int sStkAny[SD * kTravThreads];
+    __shared__ unsigned long long cTab[kMaxJobs * 2];
+    SegMap sm;
+    const unsigned n = loadSegMap(q.ctrl->mirrorCount, q.mirrorSegCap, sm);
+    const int nJobs = q.ctrl->nJobs;
+    initJobTable<2>(cTab, nJobs);
+    JobAcc<2> cacc;   // ray queries of the connections, splats
+    AccAddr ca{q.jobs, kAccPhotons + 2};
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int2* spill = SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr;
+    for (unsigned li = gtid; li < n; li += stride) {
+        const unsigned i = sm.slot(li);
+        const float4 a = q.mv[0][i], b = q.mv[1][i], l4 = q.mv[2][i], w4 = q.mv[3][i];
+        const unsigned tag = (unsigned)__float_as_int(a.w);
+        const int j = (int)(tag & 0xffffu);
+        const int nUp = (int)((tag >> 16) & 7u);
+        const DevJob& J = q.jobs[j];
+        accumulate(cacc, j, cTab);
+        const float lam[kMaxHero] = {l4.x, l4.y, l4.z, l4.w};
+        const float bw[kMaxHero] = {w4.x, w4.y, w4.z, w4.w};
+        const double p[3] = {(double)a.x, (double)a.y, (double)a.z};
+        const double np[3] = {(double)b.x, (double)b.y, (double)b.z};
+        for (int m = 0; m < J.nMirrors; ++m) {
+            const DevMirrorPlane P = J.mirrors[m];
+            for (int k = 0; k < J.nCams; ++k) {
+                const DevCamera& K = J.cams[k];
+                MirrorSolve g;
+                if (!solveMirror(P, K, p, np, g)) continue;
+                cacc.v[0] += 1;
+                const F3 wl{(float)g.wP[0], (float)g.wP[1], (float)g.wP[2]};
+                const F3 ol{(float)(p[0] + g.wP[0] * 1e-6), (float)(p[1] + g.wP[1] * 1e-6),
+                            (float)(p[2] + g.wP[2] * 1e-6)};
+                Stack<SD, false, SPILL> as{nullptr, sStkAny + threadIdx.x, spill, 0};
+                if (traceAny<SD, SPILL>(J.nodes, J.tris, prepareRay(ol, wl), kRayEps,
+                                        (float)((g.dP - 2e-6) - 1e-6), as))
+                    continue;
+                cacc.v[0] += 1;
+                Stack<SD, true, SPILL> cs{sStk + threadIdx.x, nullptr, spill, 0};
+                float th = INFINITY;
+                const int tri = traceClosest<SD, SPILL>(
+                    J.nodes, J.tris, prepareRay(load3(K.eye), F3{(float)g.wE[0], (float)g.wE[1], (float)g.wE[2]}),
+                    kRayEps, th, cs);
+                if (tri < 0 || fabs((double)th - g.dE) > 1e-4 * (1.0 + g.dE)) continue;
+                const DevMaterial& M = J.mats[__float_as_int(J.trin[tri].w)];
+                if (M.type != 2) continue;   // only Mirror is a planar mirror here
+                cacc.v[1] += 1;
+                double X = 0.0, Y = 0.0, Z = 0.0;
+                unsigned long long H = 0;
+                for (int c = 0; c < nUp; ++c) {
+                    float r = specEval(M.reflect, lam[c]);
+                    r = r < 0.0f ? 0.0f : (r > 1.0f ? 1.0f : r);
+                    const double contrib = (double)bw[c] * g.weight * (double)r;
+                    if (!(contrib > 0.0)) continue;
+                    const F3 cw = cie(lam[c]);
+                    X += (double)cw.x * contrib;
+                    Y += (double)cw.y * contrib;
+                    Z += (double)cw.z * contrib;
+                    ++H;
+                }
+                if (H) {
+                    unsigned long long* f = (unsigned long long*)K.film + 4 * (size_t)g.pix;
+                    const unsigned long long qx = min(toFixed(X, K.filmScale), K.filmQmax);
+                    const unsigned long long qy = min(toFixed(Y, K.filmScale), K.filmQmax);
+                    const unsigned long long qz = min(toFixed(Z, K.filmScale), K.filmQmax);
+                    if (qx) atomicAdd(f + 0, qx);
+                    if (qy) atomicAdd(f + 1, qy);
+                    if (qz) atomicAdd(f + 2, qz);
+                    atomicAdd(f + 3, H);
+                }
+            }
+        }
+    }
+    flushJobTable(cacc, cTab, nJobs, ca);
 }
 
 // ------------------------------------------------------------------ probes
@@ -818,6 +1012,11 @@ void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
 #define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
+    GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+#undef GP_CALL
+}
+void launchMirror(const Queues& q, const LaunchShape& ls, hipStream_t s) {
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_mirror<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
     GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
 #undef GP_CALL
 }

@@ -20,13 +20,17 @@ struct Queues {
     float4* co;      // connection origin.xyz, segment end
     float4* cd;      // connection dir.xyz, pixel index
     float4* cx;      // connection X, Y, Z, (job | nUp << 16)
+    // Vertices to connect to the cameras through flat mirrors (allocated once a scene
+    // with mirrors is traced; null before): position + (job | nUp << 16), normal,
+    // wavelengths, per-wavelength weight (beta * albedo).
+    float4* mv[4];
     Ctrl* ctrl;
     const DevJob* jobs;
     int2* spill;     // traversal stack spill, spillDepth entries per thread
     int spillDepth;
     int lds;         // LDS stack entries per lane (8, 16, 24 or 32)
     int useSpill;    // 1: the traced scenes are deeper than the LDS stack (spill variant)
-    unsigned segCap, connSegCap;
+    unsigned segCap, connSegCap, mirrorSegCap;
 };
 
 struct LaunchShape {
@@ -39,6 +43,7 @@ void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s);
+void launchMirror(const Queues& q, const LaunchShape& ls, hipStream_t s);
 
 // Batch probes over host-provided rays (already on the device).
 void launchProbeClosest(const float4* nodes, const float4* tris, const float* rays, long long n,

@@ -205,7 +205,9 @@ double sceneMinDistance(const gp_scene& s, const double p[3]) {
 // most N * (maxDepth + 1) * 2^K <= 2^62 in any channel: no int64 overflow.
 gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
                       const gp_trace_params& p) {
-    const double splats = (double)p.n_photons * (double)(p.max_depth + 1);
+    // every vertex splats at most once directly and once through each mirror plane
+    const double perVertex = 1.0 + (double)s->nMirrors;
+    const double splats = (double)p.n_photons * (double)(p.max_depth + 1) * perVertex;
     if (film->scale > 0.0) {
         if (film->splatUsed + splats > film->splatBudget * (1.0 + 1e-12))
             return fail(GP_ERR_CAPACITY, "film photon budget exceeded (create it with a larger max_photons)");
@@ -217,7 +219,8 @@ gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
     const double cieMax = 1.8;   // max over [360, 830] nm of the three fitted CIE curves is < 1.8
     const double B = s->basePower * cieMax / 3.141592653589793 /
                      (dmin * dmin * cb.pixelArea * cosMin * cosMin * cosMin);
-    const double budget = std::max((double)film->maxPhotons, (double)p.n_photons) * (double)(p.max_depth + 1);
+    const double budget =
+        std::max((double)film->maxPhotons, (double)p.n_photons) * (double)(p.max_depth + 1) * perVertex;
     double K = std::floor(62.0 - std::log2(budget) - 1e-9);
     K = std::min(K, 52.0);
     if (K < 8.0) return fail(GP_ERR_CAPACITY, "photon budget too large for a 64-bit film");
@@ -271,6 +274,8 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const*
     J.basePower = (float)s->basePower;
     J.cams = E.dJobCams + (size_t)slot * kMaxCams;
     J.nCams = nCams;
+    J.nMirrors = s->nMirrors;
+    J.mirrors = s->dMirrors;
     J.acc = E.dJobAcc + (size_t)slot * kSegs * kAccRow;
     const double ebits = std::floor(60.0 - std::log2(std::max(1.0, (double)p.n_photons)) - 2.0);
     J.energyScale = std::ldexp(1.0, (int)std::max(8.0, ebits)) / std::max(s->basePower, 1e-300);
@@ -360,6 +365,7 @@ void Engine::initCtrl() {
     hCtrl->capacity = capacity;
     hCtrl->segCap = segCap;
     hCtrl->connSegCap = 2 * segCap * (unsigned)camCap;
+    hCtrl->mirrorSegCap = 2 * segCap;
     for (int k = 0; k < kMaxJobs; ++k) hCtrl->lastEmitIter[k] = -1;
 }
 
@@ -383,7 +389,7 @@ gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned lo
     return GP_OK;
 }
 
-gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams) {
+gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams, bool mirrors) {
     if (!tunSet) { tun = defaultTuning(); tunSet = true; }
     DeviceState& d = device();
     const int cus = d.cus > 0 ? d.cus : 256;
@@ -395,12 +401,14 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams) {
     const unsigned sc = segmentBound(cap, (unsigned)ls.blocksShade);
     const int camsNeed = std::max(camCap, std::max(1, cams));
     const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
+    const bool wantMirror = mirrors || mirrorQueue;
+    const bool needMirrorAlloc = wantMirror && (needQueues || !q.mv[0]);
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run
     // when every traced scene fits (needSpill == 0). A change of variant re-captures
     // the graph.
     const int useSpill = needSpill > 0 ? 1 : 0;
-    if (!needQueues && !needSpillAlloc && q.lds == lds && q.useSpill == useSpill &&
+    if (!needQueues && !needSpillAlloc && !needMirrorAlloc && q.lds == lds && q.useSpill == useSpill &&
         graphShape == ls.blocksExtend)
         return GP_OK;
     if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
@@ -444,7 +452,26 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams) {
         q.jobs = dJobs;
         q.segCap = sc;
         q.connSegCap = 2 * sc * (unsigned)camsNeed;
+        q.mirrorSegCap = 2 * sc;
         camCap = camsNeed;
+    }
+    if (needMirrorAlloc) {
+        // one vertex per emitted photon and per shaded path, like the connections of
+        // one camera
+        const size_t mcap = 2 * (size_t)kSegs * segCap;
+        float4* mv[4] = {};
+        for (int k = 0; k < 4; ++k) {
+            hipError_t e = hipMalloc(&mv[k], mcap * sizeof(float4));
+            if (e != hipSuccess) {
+                for (int i = 0; i < k; ++i) hipFree(mv[i]);
+                return fail(GP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            q.mv[k] = mv[k];
+            bufs[nbufs++] = mv[k];
+        }
+        mirrorQueue = true;
     }
     if (needSpillAlloc) {
         const int depthAlloc = std::max(needSpill, spillDepth);
@@ -484,6 +511,8 @@ gp_status Engine::enqueueIterations(int n) {
         if ((st = timed(2, [&] { launchExtend(q, cur, ls, s); })) != GP_OK) return st;
         if ((st = timed(3, [&] { launchShade(q, cur, ls, s); })) != GP_OK) return st;
         if ((st = timed(4, [&] { launchConnect(q, ls, s); })) != GP_OK) return st;
+        if (mirrorActive && q.mv[0])
+            if ((st = timed(4, [&] { launchMirror(q, ls, s); })) != GP_OK) return st;
         parity ^= 1;
     }
     GP_HIP(hipGetLastError());
@@ -518,7 +547,7 @@ gp_status Engine::runIterations(int n) {
     hipStream_t s = device().stream;
     const int K = chunkIterations(tun);
     if (!tun.use_graph || tun.profile || parity != 0 || n < K) return enqueueIterations(n);
-    if (!graph || graphIters != K || graphLds != q.lds) {
+    if (!graph || graphIters != K || graphLds != q.lds || graphMirror != mirrorActive) {
         if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
         hipGraph_t g;
         GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -533,6 +562,7 @@ gp_status Engine::runIterations(int n) {
         if (ie != hipSuccess) { graph = nullptr; return fail(GP_ERR_HIP, "hipGraphInstantiate failed"); }
         graphIters = K;
         graphLds = q.lds;
+        graphMirror = mirrorActive;
     }
     int done = 0;
     while (n - done >= K) {

@@ -24,6 +24,8 @@ struct gp_scene {
     gp::DevEmitter* dEms = nullptr;
     double* dCdfs = nullptr;
     double* dEmCdf = nullptr;
+    gp::DevMirrorPlane* dMirrors = nullptr;   // flat mirror planes (camera connections through them)
+    int nMirrors = 0;
     int nEm = 0;
     double totalPower = 0.0;
     double basePower = 0.0;
@@ -105,13 +107,18 @@ struct Engine {
     gp_status resetJobAcc(int slot);             // zero a slot's rows (stream-ordered)
     gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]);
     void initCtrl();                            // zero the host control block, set capacities
-    void* bufs[16] = {};
+    void* bufs[24] = {};
     int nbufs = 0;
     int parity = 0;
     hipGraphExec_t graph = nullptr;
     int graphIters = 0;
     int graphLds = 0;
     int graphShape = 0;
+    // k_mirror runs in each iteration while a job of a scene with flat mirrors is live
+    // (set by the caller before runIterations; the graph is re-captured on a change)
+    bool mirrorActive = false;
+    bool graphMirror = false;
+    bool mirrorQueue = false;           // the mirror-vertex queue has been requested once
     // per-kernel HIP-event timing (tuning.profile)
     std::vector<std::array<hipEvent_t, 2>> pending[5];
     double kernelMs[5] = {};
@@ -119,7 +126,8 @@ struct Engine {
     void resetProfile();
     gp_status collectProfile();
 
-    gp_status ensure(unsigned cap, int lds, int spillDepth, int cams = 1);
+    // mirrors: allocate the mirror-vertex queue (kept from then on, like the camera count)
+    gp_status ensure(unsigned cap, int lds, int spillDepth, int cams = 1, bool mirrors = false);
     void release();
     void releaseQueues();
     void releaseSpill();
@@ -142,5 +150,9 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const*
                   const gp_trace_params& p, int slot, DevJob& J, DevCamera* camsOut);
 gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
                       const gp_trace_params& p);
+
+// Flat mirror planes of a scene description: mirror triangles grouped by plane, at
+// most kMaxMirrorPlanes (the largest total areas), boxes padded.
+void buildMirrorPlanes(const gp_scene_desc* d, std::vector<DevMirrorPlane>& out);
 
 }  // namespace gp

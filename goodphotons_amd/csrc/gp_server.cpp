@@ -181,6 +181,9 @@ gp_status admit(Engine& E, gp_server* srv) {
             if (srv->slotJob[(size_t)k] < 0) { slot = k; break; }
         if (slot < 0) break;
         if (srv->liveBytes + J.footprint > srv->cfg.hbm_budget_bytes) break;
+        // a scene with mirrors needs the mirror-vertex queue and k_mirror in the
+        // iteration; both are set up at the start of a step, so wait for the next one
+        if (J.desc.scene->nMirrors > 0 && !(E.q.mv[0] && E.mirrorActive)) break;
         srv->waiting.pop_front();
         JobInstall a{};
         gp_status st = streamFilm(J.desc.camera, J.desc.params.n_photons, &J.film);
@@ -370,19 +373,25 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     gp_status st;
     int maxDepthScene = 0;
-    for (auto& kv : srv->jobs) maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
+    bool mirrors = false;
+    for (auto& kv : srv->jobs) {
+        maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
+        mirrors = mirrors || kv.second.desc.scene->nMirrors > 0;
+    }
     const int lds = E.tun.lds_stack;
     if (srv->engineReady && !srv->inflight.empty()) {
         // a change of engine shape would re-allocate under running chunks
         const int need = std::max(0, maxDepthScene + 1 - lds);
-        if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0)) {
+        if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0) || (mirrors && !E.q.mv[0]) ||
+            mirrors != E.mirrorActive) {
             while (!srv->inflight.empty())
                 if ((st = processOldest(E, srv)) != GP_OK) return st;
             GP_HIP(hipStreamSynchronize(device().stream));
         }
     }
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds))) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds), 1, mirrors)) != GP_OK) return st;
+    E.mirrorActive = mirrors;
     const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg;
     if (fresh) {
         if (!srv->order.empty() || !srv->inflight.empty())

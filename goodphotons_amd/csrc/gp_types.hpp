@@ -7,6 +7,7 @@
 //              into kSegs segments, each with its own append counter
 //   hits       one float2 per path (t, leaf-order triangle index)
 //   camera     connections: 3 x float4 per record (48 B)
+//   mirrors    vertices to connect through flat mirrors: 4 x float4 per record (64 B)
 //   films      int64 fixed-point (X, Y, Z, hits) per pixel, 32 B
 #pragma once
 #include <hip/hip_runtime.h>
@@ -57,6 +58,16 @@ struct DevEmitter {
 };
 
 constexpr int kMaxCams = 16;          // cameras one job splats into (shared photon flight)
+constexpr int kMaxMirrorPlanes = 64;  // flat mirror surfaces one scene connects through
+
+// One flat mirror surface: the coplanar mirror triangles of a plane, reduced to the
+// plane (unit normal n, dot(n, x) = d) and the padded box of its triangles. The
+// connection geometry is solved in double, so the plane is stored in double.
+struct DevMirrorPlane {
+    double n[3];
+    double d;
+    double lo[3], hi[3];
+};  // 80 B
 
 // Pinhole camera and the film it splats into (Camera::lookAt basis, float).
 struct DevCamera {
@@ -85,7 +96,8 @@ struct DevJob {
     // cameras
     const DevCamera* cams;   // nCams entries
     int32_t nCams;
-    int32_t pad1;
+    int32_t nMirrors;        // flat mirror planes of the scene (0: no mirror connections)
+    const DevMirrorPlane* mirrors;
     // kSegs rows of kAccRow u64: [0..4] energy (emitted, absorbed, sensor, escaped,
     // residual, fixed point), [5..8] counters (photons, segments, connections,
     // splats); workgroup b adds into row b % kSegs
@@ -106,6 +118,7 @@ inline constexpr unsigned connTag(unsigned job, unsigned nUp, unsigned cam) {
 struct Ctrl {
     unsigned int pathCount[2][kSegs][kLine];   // per segment of path queue 0 / 1 (one line each)
     unsigned int connCount[kSegs][kLine];      // per segment of the connection queue
+    unsigned int mirrorCount[kSegs][kLine];    // per segment of the mirror-vertex queue
     unsigned int nEmit;         // photons emitted this iteration
     unsigned int iter;          // iterations run
     int32_t nJobs;              // job slots in use are below this
@@ -113,7 +126,7 @@ struct Ctrl {
     unsigned int segCap;        // path slots per segment
     unsigned int connSegCap;    // connection slots per segment
     unsigned int overflow;      // set by a kernel that found a segment full (never expected)
-    unsigned int pad;
+    unsigned int mirrorSegCap;  // mirror-vertex slots per segment
     unsigned int segBase[kSegs];        // survivors already in each segment (emission goes after)
     unsigned int segEmit[kSegs + 1];    // prefix of this iteration's emission over segments
     unsigned int pad2[7];

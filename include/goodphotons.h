@@ -15,6 +15,9 @@
  *   gp_scene_create_ex     <- the same, with the BVH built on the device instead of
  *                             Bvh::build (src/bvh.h:89-176) on the host
  *   gp_scene_destroy       <- freeUpload (src/render_cuda.cu:12683)
+ *   gp_mirror_planes       <- Scene::buildMirrorPlanes / Scene::mirrorPlanes
+ *                             (src/scene.h:1606-1724), which the device upload
+ *                             copies verbatim (src/render_cuda.cu:12735-12745)
  *   gp_film_*              <- the device film inside renderForwardCuda
  *                             (src/render_cuda.cu:13722-13746) and the resident
  *                             films of SharedGpuSession (src/render_cuda.h:121-143)
@@ -101,6 +104,14 @@ enum {
     GP_MAT_GLOSSY = 4
 };
 
+/* Mirror is a perfect (delta) reflector. A forward light tracer cannot reach the
+ * camera from a delta lobe, so, as the reference does, every diffuse or emitter
+ * vertex is also connected to the camera THROUGH each flat mirror surface: the
+ * coplanar Mirror triangles of a plane (at most 64 planes, gp_mirror_planes). One
+ * specular vertex per connection (a mirror seen in a mirror stays dark), pinhole
+ * camera only. Glossy with roughness <= 0 and the reference's HalfMirror are not
+ * supported. In this release gp_scene_create still rejects Mirror with
+ * GP_ERR_UNSUPPORTED: the connection is built but not yet verified on the device. */
 typedef struct gp_material {
     int32_t type;
     int32_t reserved;
@@ -159,7 +170,9 @@ typedef struct gp_trace_params {
 typedef struct gp_trace_stats {
     uint64_t photons;        /* photons emitted                                      */
     uint64_t segments;       /* closest-hit queries (one per bounce-loop iteration)  */
-    uint64_t connections;    /* camera shadow rays cast (any-hit queries)            */
+    uint64_t connections;    /* connection ray queries: camera shadow rays, plus the
+                                light-side any hit and eye-side closest hit of each
+                                connection through a flat mirror                     */
     uint64_t splats;         /* connections that reached the film                    */
     uint64_t iterations;     /* wavefront iterations run                             */
     double wall_ms;          /* host wall time of the trace loop                     */
@@ -213,6 +226,13 @@ gp_status gp_scene_create(const gp_scene_desc* desc, gp_scene** out);
 #define GP_BVH_DEVICE_LBVH 1
 gp_status gp_scene_create_ex(const gp_scene_desc* desc, int32_t bvh_builder, gp_scene** out);
 gp_status gp_scene_get_info(const gp_scene* scene, gp_scene_info* out);
+/* Flat mirror planes the camera connection goes through, as gp_scene_create
+ * derives them from the descriptor (host only, no device needed): Mirror
+ * triangles grouped by plane, 10 doubles per plane written to out10 (unit normal
+ * n.xyz with its largest component positive, offset d with dot(n, x) = d, padded
+ * box lo.xyz, hi.xyz) for up to `cap` planes. Returns the number of planes (0..64),
+ * or -GP_ERR_INVALID. */
+int32_t gp_mirror_planes(const gp_scene_desc* desc, double* out10, int32_t cap);
 void gp_scene_destroy(gp_scene* scene);
 
 /* ---- films -------------------------------------------------------------- */

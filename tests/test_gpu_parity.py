@@ -72,10 +72,14 @@ FILM_CASES = [
     ("c0_rr0", lambda: scenes.cornell_c0(64), 1 << 18, 8, 0),
     ("c1", lambda: scenes.caustics_c1(64), 1 << 18, 16, 0),
     ("glossy", lambda: scenes.glossy_box(64), 1 << 17, 8, 0),
+    pytest.param("mirror", lambda: scenes.mirror_box(64), 1 << 18, 8, 0,
+                 marks=pytest.mark.skip(reason="flat-mirror connection switched off until its LDS-stack fix "
+                                               "is verified on the GPU")),
 ]
 
 
-@pytest.mark.parametrize("name,builder,n,depth,rr", FILM_CASES, ids=[c[0] for c in FILM_CASES])
+@pytest.mark.parametrize("name,builder,n,depth,rr", FILM_CASES,
+                         ids=["c0", "c0_rr0", "c1", "glossy", "mirror"])
 def test_film_matches_oracle(gplib, orc, name, builder, n, depth, rr):
     sd = builder()
     print(f"[film] {name} n={n} depth={depth} rr={rr}", flush=True)
@@ -146,10 +150,13 @@ def test_queue_capacity_does_not_change_result(gplib):
         gplib.set_tuning(**{k: getattr(old, k) for k, _ in old._fields_})
 
 
-def test_unsupported_material_rejected(gplib):
+@pytest.mark.parametrize("mtype,rough", [(abi.MAT_MIRROR, 0.0), (3, 0.0), (abi.MAT_GLOSSY, 0.0), (7, 0.0)],
+                         ids=["mirror_switched_off", "halfmirror", "glossy_rough0", "grating"])
+def test_unsupported_material_rejected(gplib, mtype, rough):
     sd = scenes.cornell_c0(32)
     m = abi.Material()
-    m.type = abi.MAT_MIRROR
+    m.type = mtype
+    m.roughness = rough
     m.reflect = abi.spectrum(abi.SPEC_CONST, 0.9)
     sd.materials[0] = m
     with pytest.raises(gp.GpError) as ei:

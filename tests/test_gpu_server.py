@@ -11,6 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _jobs():
+    # (scenes.mirror_box(32), 1 << 15, 8, 0, 55) joins this mix once the flat-mirror
+    # connection is switched back on
     return [
         (scenes.cornell_c0(64), 1 << 16, 8, 3, 11),
         (scenes.caustics_c1(48), 1 << 16, 16, 0, 22),
