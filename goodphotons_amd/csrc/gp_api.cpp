@@ -4,6 +4,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -135,6 +136,13 @@ gp_status gp_device_query(gp_device_info* out) {
 }
 
 // ------------------------------------------------------------------ scenes
+// Mirror materials stay rejected until the flat-mirror connection has run on the GPU;
+// GP_EXPERIMENTAL_MIRROR=1 in the environment admits them (CPU checks of k_mirror).
+static bool experimentalMirror() {
+    const char* v = std::getenv("GP_EXPERIMENTAL_MIRROR");
+    return v && v[0] == '1' && v[1] == 0;
+}
+
 static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out) {
     if (!d || !out) return fail(GP_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -144,7 +152,7 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
     if (d->n_tris >= (int64_t(1) << 28)) return fail(GP_ERR_CAPACITY, "more than 2^28 triangles");
     for (int i = 0; i < d->n_materials; ++i) {
         const gp_material& m = d->materials[i];
-        if (m.type == GP_MAT_MIRROR)
+        if (m.type == GP_MAT_MIRROR && !experimentalMirror())
             return fail(GP_ERR_UNSUPPORTED,
                         "Mirror material: the flat-mirror camera connection is switched off until its "
                         "fix (separate LDS stacks for its two traversals) is verified on the GPU");

@@ -14,7 +14,8 @@ from goodphotons_amd import scenes  # noqa: E402
 name, cap, n, res = sys.argv[1], int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])
 lib = gp.Library(os.path.join(ROOT, "tests", "emu", "_build", "libgp_emu.so"))
 lib.set_tuning(queue_capacity=cap)
-sd = {"c0": scenes.cornell_c0, "c1": scenes.caustics_c1, "glossy": scenes.glossy_box}[name](res)
+sd = {"c0": scenes.cornell_c0, "c1": scenes.caustics_c1, "glossy": scenes.glossy_box,
+      "mirror": scenes.mirror_box}[name](res)
 gs = gp.GpuScene(sd, lib)
 x, h, e, st = gp.render_forward(gs, sd.camera, n, hero_c=4, max_depth=8)
 print(json.dumps({"stats": st, "energy": e, "xyz": x.tolist(), "hits": h.tolist()}))

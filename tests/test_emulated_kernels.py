@@ -5,6 +5,9 @@ memory access of the device code is bounds-checked without a GPU.
 - A small trace with lanes preempted inside their loops (so lanes of a block interleave
   as waves do) traces exactly the oracle's photons: equal segment, connection and splat
   counts and the same film up to float rounding.
+- The flat-mirror connection (k_mirror, admitted here with GP_EXPERIMENTAL_MIRROR=1)
+  with interleaved lanes: the configuration in which its old shared LDS stack made
+  lanes pop garbage nodes; it must trace the oracle's photons.
 - Regression for the splat-table race in k_connect: at 128^2 with a 2^14-path queue on
   a 4-workgroup grid, the old table re-insertion (no barrier after it) indexed the LDS
   table with -1 and sent a splat to a garbage film address here; it must run clean.
@@ -31,10 +34,12 @@ def emulated():
     return emu.build()
 
 
-def _run(name, cap, n, res, cus, preempt=0):
+def _run(name, cap, n, res, cus, preempt=0, mirror=False):
     env = dict(os.environ)
     env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0",
                 "EMU_CUS": str(cus), "EMU_PREEMPT": str(preempt)})
+    if mirror:
+        env["GP_EXPERIMENTAL_MIRROR"] = "1"
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_trace.py"), name,
                         str(cap), str(n), str(res)], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -43,7 +48,8 @@ def _run(name, cap, n, res, cus, preempt=0):
 
 
 def _oracle(orc, name, n, res):
-    sd = {"c0": scenes.cornell_c0, "c1": scenes.caustics_c1, "glossy": scenes.glossy_box}[name](res)
+    sd = {"c0": scenes.cornell_c0, "c1": scenes.caustics_c1, "glossy": scenes.glossy_box,
+          "mirror": scenes.mirror_box}[name](res)
     return orc.render(orc.scene(sd), sd.camera, n, hero_c=4, max_bounce=8, threads=8)
 
 
@@ -52,6 +58,14 @@ def test_interleaved_lanes_trace_the_oracles_photons(emulated, orc):
     cx, ch, ce, cst = _oracle(orc, "c1", 4096, 32)
     assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
     m = image_metrics(x, cx, 4096, 4096, res=(32, 32))
+    assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
+
+
+def test_mirror_connection_with_interleaved_lanes(emulated, orc):
+    x, h, e, st = _run("mirror", 4096, 2048, 32, cus=2, preempt=3, mirror=True)
+    cx, ch, ce, cst = _oracle(orc, "mirror", 2048, 32)
+    assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
+    m = image_metrics(x, cx, 2048, 2048, res=(32, 32))
     assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
 
 
