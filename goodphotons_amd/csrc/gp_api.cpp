@@ -242,8 +242,12 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         s->nNodes = (int64_t)(bvh.pairs.size() / 16);
         s->depth = bvh.depth;
         s->buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        s->leafTris.resize(9 * n);
-        for (size_t k = 0; k < n; ++k) {
+        // a triangle may be referenced from several leaves (one entry per reference)
+        const size_t m = bvh.order.size();
+        tris.resize(3 * m);
+        trin.resize(m);
+        s->leafTris.resize(9 * m);
+        for (size_t k = 0; k < m; ++k) {
             const int prim = bvh.order[k];
             fillTri(k, prim);
             std::memcpy(&s->leafTris[9 * k], d->verts + 9 * (size_t)prim, 9 * sizeof(float));
