@@ -429,7 +429,10 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (st != GP_OK) return st;
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
+    if (E.owner)
+        return fail(GP_ERR_STATE, "a server has jobs in flight on this device; step it until they finish first");
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
+    ++E.epoch;
     const int lds = E.tun.lds_stack;
     const int needSpill = std::max(0, s->depth + 1 - lds);
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams, s->nMirrors > 0)) != GP_OK) return st;
@@ -603,6 +606,7 @@ gp_status gp_set_tuning(const gp_tuning* in) {
         return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
+    if (E.owner) return fail(GP_ERR_STATE, "a server has jobs in flight on this device");
     E.tun = *in;
     E.tun.profile = in->profile ? 1 : 0;
     E.tun.block_extend = 256;

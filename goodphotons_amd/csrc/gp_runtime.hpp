@@ -119,6 +119,12 @@ struct Engine {
     bool mirrorActive = false;
     bool graphMirror = false;
     bool mirrorQueue = false;           // the mirror-vertex queue has been requested once
+    // The control block holds one trace or one server's jobs at a time. `owner` is the
+    // server whose jobs are in the engine (live or with chunks in flight); `epoch` counts
+    // re-initialisations, so a server sees that a trace used the engine since its last
+    // step and starts from a fresh control block.
+    const void* owner = nullptr;
+    uint64_t epoch = 0;
     // per-kernel HIP-event timing (tuning.profile)
     std::vector<std::array<hipEvent_t, 2>> pending[5];
     double kernelMs[5] = {};

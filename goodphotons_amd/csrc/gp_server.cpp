@@ -94,6 +94,7 @@ struct gp_server {
     int64_t liveBytes = 0;
     int64_t finished = 0;
     bool engineReady = false;
+    uint64_t epoch = 0;                 // engine epoch of this server's control block
     // control-block snapshots of the chunks in flight
     struct Chunk {
         Ctrl* snap = nullptr;
@@ -370,6 +371,7 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     if (!srv) return fail(GP_ERR_INVALID, "null server");
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
+    if (E.owner && E.owner != srv) return fail(GP_ERR_STATE, "another server has jobs in flight on this device");
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     gp_status st;
     int maxDepthScene = 0;
@@ -392,7 +394,7 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds), 1, mirrors)) != GP_OK) return st;
     E.mirrorActive = mirrors;
-    const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg;
+    const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg || srv->epoch != E.epoch;
     if (fresh) {
         if (!srv->order.empty() || !srv->inflight.empty())
             return fail(GP_ERR_STATE, "engine re-allocated under a running server");
@@ -400,7 +402,9 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
         GP_HIP(hipMemcpy(E.dCtrl, E.hCtrl, sizeof(Ctrl), hipMemcpyHostToDevice));
         E.parity = 0;
         srv->engineReady = true;
+        srv->epoch = ++E.epoch;
     }
+    E.owner = srv;   // recomputed when the step ends
     const int K = chunkIterations(E.tun);
     int64_t ran = 0;
     while (true) {
@@ -424,6 +428,8 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
         if (maxIterations > 0 && ran >= maxIterations) break;
     }
     if (maxIterations <= 0 && (st = pollRetiring(srv, true)) != GP_OK) return st;
+    // the engine stays this server's while it holds live jobs or chunks in flight
+    E.owner = (srv->order.empty() && srv->inflight.empty()) ? nullptr : srv;
     if (finishedOut) *finishedOut = (int32_t)srv->finished;
     return GP_OK;
 }
@@ -525,6 +531,7 @@ void gp_server_destroy(gp_server* srv) {
         if (c.ev) hipEventDestroy(c.ev);
     }
     for (hipEvent_t e : srv->eventPool) hipEventDestroy(e);
+    if (E.owner == srv) E.owner = nullptr;
     delete srv;
 }
 

@@ -360,6 +360,9 @@ typedef struct gp_job_result {
     double submit_ms, start_ms, finish_ms;   /* server clock, ms                     */
 } gp_job_result;
 
+/* A server is driven from one host thread at a time: submit, step, poll and take
+ * may be called in any order but not concurrently on the same server. All servers and
+ * traces of a process share one device engine (gp_trace waits for it). */
 gp_status gp_server_create(const gp_server_config* cfg, gp_server** out);
 /* Queue a job. It is admitted into the running wavefront as soon as a job slot
  * and its HBM footprint are free. Returns its id in *job_id. */

@@ -100,3 +100,62 @@ def test_admission_by_hbm_budget(gplib):
         srv.submit(gs, big.camera, 100, max_depth=8)
     assert ei.value.status == 7
     srv.close()
+
+
+def test_engine_is_exclusive_while_a_server_has_jobs_in_flight(gplib):
+    # One control block per device: while a server holds live jobs, a trace, a tuning
+    # change or a second server is refused (GP_ERR_STATE) instead of clobbering them;
+    # once the server has drained, traces work again and the server's result is unchanged.
+    sd = scenes.cornell_c0(32)
+    gs = gp.GpuScene(sd, gplib)
+    ref = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=4, lib=gplib)
+    rj = ref.submit(gs, sd.camera, 1 << 22, seed_base=3, max_depth=8)
+    ref.step()
+    r0, x0, h0 = ref.take(rj, 32, 32)
+    ref.close()
+    srv = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=4, lib=gplib)
+    jid = srv.submit(gs, sd.camera, 1 << 22, seed_base=3, max_depth=8)
+    srv.step(max_iterations=4)
+    assert srv.pending()[1] == 1
+    f = gp.Film(32, 32, max_photons=1 << 12, lib=gplib)
+    with pytest.raises(gp.GpError) as ei:
+        gp.trace(gs, sd.camera, f, 1 << 12, max_depth=8)
+    assert ei.value.status == 8
+    with pytest.raises(gp.GpError) as ei:
+        gplib.set_tuning(iters_per_graph=8)
+    assert ei.value.status == 8
+    other = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=4, lib=gplib)
+    other.submit(gs, sd.camera, 1 << 12, seed_base=9, max_depth=8)
+    with pytest.raises(gp.GpError) as ei:
+        other.step()
+    assert ei.value.status == 8
+    srv.step()
+    r1, x1, h1 = srv.take(jid, 32, 32)
+    assert np.array_equal(x0, x1) and np.array_equal(h0, h1)
+    gp.trace(gs, sd.camera, f, 1 << 12, max_depth=8)      # the device is free again
+    other.step()
+    other.close()
+    f.close()
+    srv.close()
+
+
+def test_server_resumes_after_a_trace_used_the_engine(gplib):
+    # A trace between two of a server's jobs resets the shared control block; the server
+    # notices (engine epoch) and starts its next job from a fresh one.
+    sd = scenes.cornell_c0(32)
+    gs = gp.GpuScene(sd, gplib)
+    alone = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=4, lib=gplib)
+    aj = alone.submit(gs, sd.camera, 1 << 16, seed_base=77, max_depth=8)
+    alone.step()
+    ra, xa, ha = alone.take(aj, 32, 32)
+    alone.close()
+    srv = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=4, lib=gplib)
+    j1 = srv.submit(gs, sd.camera, 1 << 16, seed_base=5, max_depth=8)
+    srv.step()
+    srv.take(j1, 32, 32)
+    gp.render_forward(gs, sd.camera, 1 << 16, seed_base=11, max_depth=8)
+    j2 = srv.submit(gs, sd.camera, 1 << 16, seed_base=77, max_depth=8)
+    srv.step()
+    rb, xb, hb = srv.take(j2, 32, 32)
+    srv.close()
+    assert np.array_equal(xa, xb) and np.array_equal(ha, hb)
