@@ -377,6 +377,7 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     int maxDepthScene = 0;
     bool mirrors = false;
     for (auto& kv : srv->jobs) {
+        if (kv.second.state == JobState::Finished) continue;   // its scene is no longer used
         maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
         mirrors = mirrors || kv.second.desc.scene->nMirrors > 0;
     }

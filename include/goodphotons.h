@@ -234,8 +234,8 @@ gp_status gp_scene_get_info(const gp_scene* scene, gp_scene_info* out);
  * box lo.xyz, hi.xyz) for up to `cap` planes. Returns the number of planes (0..64),
  * or -GP_ERR_INVALID. */
 int32_t gp_mirror_planes(const gp_scene_desc* desc, double* out10, int32_t cap);
-/* A scene must outlive every trace and server job that uses it (a server job holds
- * the scene until gp_server_take_result or gp_server_destroy). */
+/* A scene must outlive every trace and server job that uses it (a server job uses
+ * its scene until the job has finished). */
 void gp_scene_destroy(gp_scene* scene);
 
 /* ---- films -------------------------------------------------------------- */
