@@ -8,6 +8,8 @@
 //                          survivors compacted into the other queue by wave ballot
 //   k_connect              any-hit for every connection; unoccluded ones splat into
 //                          the int64 fixed-point film with 64-bit atomics
+//   k_mirror               (scenes with flat mirrors only) camera connections of the
+//                          queued vertices through each mirror plane
 // All kernels are fixed-grid and grid-stride over counts that live on the device,
 // so an iteration needs no host round trip and can be captured in a hipGraph.
 //
