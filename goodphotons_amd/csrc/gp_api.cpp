@@ -96,6 +96,16 @@ struct DevBuf {
 
 using namespace gp;
 
+gp_scene::~gp_scene() {
+    void* ptrs[] = {dNodes, dTris, dTrin, dMats, dEms, dCdfs, dEmCdf, dMirrors};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+}
+
+gp_film::~gp_film() {
+    if (d) hipFree(d);
+}
+
 extern "C" {
 
 int32_t gp_abi_version(void) { return GP_ABI_VERSION; }
@@ -287,20 +297,23 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
 
     auto upload = [&](const void* src, size_t bytes, void** dst) -> gp_status {
         hipError_t e = hipMalloc(dst, bytes);
-        if (e != hipSuccess) return fail(GP_ERR_OOM, "scene upload: hipMalloc failed");
+        if (e != hipSuccess) {
+            *dst = nullptr;
+            return fail(GP_ERR_OOM, "scene upload: hipMalloc failed");
+        }
         e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) return fail(GP_ERR_HIP, std::string("scene upload: ") + hipGetErrorString(e));
         s->deviceBytes += (int64_t)bytes;
         return GP_OK;
     };
     if (builder != GP_BVH_DEVICE_LBVH) {
-        if ((st = upload(s->pairs.data(), s->pairs.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-        if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-        if ((st = upload(trin.data(), trin.size() * sizeof(float4), (void**)&s->dTrin)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+        if ((st = upload(s->pairs.data(), s->pairs.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) return st;
+        if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) return st;
+        if ((st = upload(trin.data(), trin.size() * sizeof(float4), (void**)&s->dTrin)) != GP_OK) return st;
     }
-    if ((st = upload(mats.data(), mats.size() * sizeof(DevMaterial), (void**)&s->dMats)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-    if ((st = upload(cdfs.data(), cdfs.size() * sizeof(double), (void**)&s->dCdfs)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
-    if ((st = upload(emCdf.data(), emCdf.size() * sizeof(double), (void**)&s->dEmCdf)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+    if ((st = upload(mats.data(), mats.size() * sizeof(DevMaterial), (void**)&s->dMats)) != GP_OK) return st;
+    if ((st = upload(cdfs.data(), cdfs.size() * sizeof(double), (void**)&s->dCdfs)) != GP_OK) return st;
+    if ((st = upload(emCdf.data(), emCdf.size() * sizeof(double), (void**)&s->dEmCdf)) != GP_OK) return st;
     for (int i = 0; i < nEm; ++i) {
         const gp_emitter& e = d->emitters[i];
         DevEmitter& de = ems[(size_t)i];
@@ -316,11 +329,11 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         de.power = (float)powers[(size_t)i];
         de.cdf = s->dCdfs + (size_t)i * (kLambdaBins + 1);
     }
-    if ((st = upload(ems.data(), ems.size() * sizeof(DevEmitter), (void**)&s->dEms)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+    if ((st = upload(ems.data(), ems.size() * sizeof(DevEmitter), (void**)&s->dEms)) != GP_OK) return st;
     std::vector<DevMirrorPlane> planes;
     buildMirrorPlanes(d, planes);
     if (!planes.empty()) {
-        if ((st = upload(planes.data(), planes.size() * sizeof(DevMirrorPlane), (void**)&s->dMirrors)) != GP_OK) { gp_scene_destroy(s.release()); return st; }
+        if ((st = upload(planes.data(), planes.size() * sizeof(DevMirrorPlane), (void**)&s->dMirrors)) != GP_OK) return st;
         s->nMirrors = (int)planes.size();
     }
     *out = s.release();
@@ -348,13 +361,7 @@ gp_status gp_scene_get_info(const gp_scene* s, gp_scene_info* out) {
     return GP_OK;
 }
 
-void gp_scene_destroy(gp_scene* s) {
-    if (!s) return;
-    void* ptrs[] = {s->dNodes, s->dTris, s->dTrin, s->dMats, s->dEms, s->dCdfs, s->dEmCdf, s->dMirrors};
-    for (void* p : ptrs)
-        if (p) hipFree(p);
-    delete s;
-}
+void gp_scene_destroy(gp_scene* s) { delete s; }
 
 // ------------------------------------------------------------------ films
 gp_status gp_film_create(int32_t rx, int32_t ry, int64_t maxPhotons, gp_film** out) {
@@ -368,9 +375,11 @@ gp_status gp_film_create(int32_t rx, int32_t ry, int64_t maxPhotons, gp_film** o
     f->ry = ry;
     f->maxPhotons = maxPhotons > 0 ? maxPhotons : (int64_t(1) << 32);
     const size_t bytes = (size_t)rx * ry * 4 * sizeof(unsigned long long);
-    GP_HIP(hipMalloc(&f->d, bytes));
-    hipError_t e = hipMemset(f->d, 0, bytes);
-    if (e != hipSuccess) { hipFree(f->d); return fail(GP_ERR_HIP, "film clear failed"); }
+    if (hipMalloc(&f->d, bytes) != hipSuccess) {
+        f->d = nullptr;
+        return fail(GP_ERR_OOM, "film allocation failed");
+    }
+    if (hipMemset(f->d, 0, bytes) != hipSuccess) return fail(GP_ERR_HIP, "film clear failed");
     *out = f.release();
     return GP_OK;
 }
@@ -410,11 +419,7 @@ gp_status gp_film_download(gp_film* f, double* xyz, double* hits) {
     return GP_OK;
 }
 
-void gp_film_destroy(gp_film* f) {
-    if (!f) return;
-    if (f->d) hipFree(f->d);
-    delete f;
-}
+void gp_film_destroy(gp_film* f) { delete f; }
 
 // ------------------------------------------------------------------ trace
 gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cams, gp_film* const* films,

@@ -234,17 +234,37 @@ gp_status prepareFilm(gp_film* film, const gp_scene* s, const CamBasis& cb,
 gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const* films, int nCams,
                   const gp_trace_params& p, int slot, DevJob& J, DevCamera* camsOut) {
     if (nCams < 1 || nCams > kMaxCams) return fail(GP_ERR_INVALID, "1..16 cameras per job");
-    gp_status st;
+    CamBasis cbs[kMaxCams];
     for (int c = 0; c < nCams; ++c) {
-        CamBasis cb;
-        if (!cameraBasis(cams[c], cb)) return fail(GP_ERR_INVALID, "degenerate camera");
-        gp_film* film = films[c];
+        if (!cameraBasis(cams[c], cbs[c])) return fail(GP_ERR_INVALID, "degenerate camera");
+        const gp_film* film = films[c];
         if (!film) return fail(GP_ERR_INVALID, "null film");
         if (cams[c].res_x != film->rx || cams[c].res_y != film->ry)
             return fail(GP_ERR_INVALID, "camera resolution differs from the film's");
         for (int d = 0; d < c; ++d)
             if (films[d] == film) return fail(GP_ERR_INVALID, "two cameras share one film");
-        if ((st = prepareFilm(film, s, cb, p)) != GP_OK) return st;
+    }
+    // Films take the job's photons into their budgets; if one refuses, the films
+    // before it give them back, so a refused job leaves every film as it was.
+    struct Budget { double scale, budget, used; unsigned long long qmax; };
+    Budget saved[kMaxCams];
+    for (int c = 0; c < nCams; ++c) {
+        gp_film* film = films[c];
+        saved[c] = {film->scale, film->splatBudget, film->splatUsed, film->qmax};
+        gp_status st = prepareFilm(film, s, cbs[c], p);
+        if (st != GP_OK) {
+            for (int d = 0; d <= c; ++d) {
+                films[d]->scale = saved[d].scale;
+                films[d]->splatBudget = saved[d].budget;
+                films[d]->splatUsed = saved[d].used;
+                films[d]->qmax = saved[d].qmax;
+            }
+            return st;
+        }
+    }
+    for (int c = 0; c < nCams; ++c) {
+        const CamBasis& cb = cbs[c];
+        gp_film* film = films[c];
         DevCamera& K = camsOut[c];
         std::memset(&K, 0, sizeof K);
         for (int i = 0; i < 3; ++i) {

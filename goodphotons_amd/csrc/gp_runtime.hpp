@@ -38,6 +38,11 @@ struct gp_scene {
     std::vector<std::array<double, 12>> emitQuads;   // origin, u, v, normal
     int builder = GP_BVH_HOST_SAH;
     mutable std::map<std::array<double, 3>, double> eyeDist2;
+
+    gp_scene() = default;
+    gp_scene(const gp_scene&) = delete;
+    gp_scene& operator=(const gp_scene&) = delete;
+    ~gp_scene();   // frees the device tables
 };
 
 struct gp_film {
@@ -48,6 +53,11 @@ struct gp_film {
     unsigned long long qmax = 0;        // largest single splat value
     double splatBudget = 0.0;           // photons*(maxDepth+1) the scale was sized for
     double splatUsed = 0.0;
+
+    gp_film() = default;
+    gp_film(const gp_film&) = delete;
+    gp_film& operator=(const gp_film&) = delete;
+    ~gp_film();    // frees d (a stream-ordered film clears d after its asynchronous free)
 };
 
 namespace gp {

@@ -548,6 +548,12 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
     const int ldss[] = {8, 16};
     double bestRate = -1.0;
     gp_tuning bestT = base;
+    // a failed trial leaves the caller's tuning in place
+    struct Restore {
+        const gp_tuning& t;
+        bool armed = true;
+        ~Restore() { if (armed) gp_set_tuning(&t); }
+    } restore{base};
     for (int cap : caps)
         for (int w : waves)
             for (int l : ldss) {
@@ -570,6 +576,7 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
                 if (rate > bestRate) { bestRate = rate; bestT = t; }
             }
     if ((st = gp_set_tuning(&bestT)) != GP_OK) return st;
+    restore.armed = false;
     if (best) *best = bestT;
     return GP_OK;
 }

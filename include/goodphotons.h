@@ -334,7 +334,7 @@ typedef struct gp_tuning {
 gp_status gp_get_tuning(gp_tuning* out);
 gp_status gp_set_tuning(const gp_tuning* in);
 /* Times candidate settings on `scene`/`cam` with a short trace and installs
- * the fastest. */
+ * the fastest. On failure the tuning in place before the call is restored. */
 gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam,
                       const gp_trace_params* params, gp_tuning* best_out);
 

@@ -58,3 +58,19 @@ def test_shared_rejects_bad_arguments(gplib):
     with pytest.raises(gp.GpError):          # more than 16 cameras
         films = [gp.Film(16, 16, max_photons=1 << 10, lib=gplib) for _ in range(17)]
         gp.trace_shared(gs, [sd.camera] * 17, films, 1 << 10, max_depth=8)
+
+
+def test_refused_shared_trace_leaves_every_film_unchanged(gplib):
+    """A job whose second film refuses its photons (budget used up) must not charge the
+    first film: that film still takes its full budget afterwards."""
+    sd = scenes.cornell_c0(16)
+    gs = gp.GpuScene(sd, gplib)
+    cam2 = _cams(sd)[1]
+    n = 1 << 12
+    fa = gp.Film(16, 16, max_photons=n, lib=gplib)
+    fb = gp.Film(cam2.res_x, cam2.res_y, max_photons=n, lib=gplib)
+    gp.trace(gs, cam2, fb, n, max_depth=8)                     # fb's budget is now spent
+    with pytest.raises(gp.GpError) as err:
+        gp.trace_shared(gs, [sd.camera, cam2], [fa, fb], n, max_depth=8)
+    assert gp.abi.STATUS_NAMES[err.value.status] == "GP_ERR_CAPACITY"
+    gp.trace(gs, sd.camera, fa, n, max_depth=8)                # fa still holds n photons
