@@ -364,7 +364,12 @@ typedef struct gp_job_result {
 
 /* A server is driven from one host thread at a time: submit, step, poll and take
  * may be called in any order but not concurrently on the same server. All servers and
- * traces of a process share one device engine (gp_trace waits for it). */
+ * traces of a process share one device engine. While a server holds live jobs or
+ * chunks in flight the engine is that server's: gp_trace, gp_trace_shared,
+ * gp_set_tuning, gp_autotune and other servers' steps return GP_ERR_STATE until it has
+ * drained (a step with max_iterations = 0) or is destroyed. A step that fails (e.g.
+ * GP_ERR_CAPACITY on a queue overflow) leaves the server's live jobs undefined:
+ * destroy the server, which releases the engine. */
 gp_status gp_server_create(const gp_server_config* cfg, gp_server** out);
 /* Queue a job. It is admitted into the running wavefront as soon as a job slot
  * and its HBM footprint are free. Returns its id in *job_id. */
