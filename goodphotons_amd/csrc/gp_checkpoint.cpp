@@ -15,6 +15,7 @@
 // the section (written by the reference) holds only doubles; resuming it is the
 // reference's own host-side merge of the downloaded film, which this core does not
 // replace.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -131,6 +132,13 @@ gp_status gp_film_load_checkpoint(const char* path, int32_t mode, uint64_t guard
     if (!get(f.get(), &fscale, 8) || !get(f.get(), &qmax, 8) || !get(f.get(), &budget, 8) ||
         !get(f.get(), &used, 8) || !get(f.get(), &maxPh, 8))
         return fail(GP_ERR_INVALID, std::string(path) + " is truncated");
+    // a film that has been traced into has a positive scale, a power-of-two saturation
+    // value of at most 2^52 and a budget covering what it used; a fresh one has none
+    const bool fresh = fscale == 0.0 && qmax == 0 && budget == 0.0 && used == 0.0;
+    const bool traced = std::isfinite(fscale) && fscale > 0.0 && qmax != 0 && (qmax & (qmax - 1)) == 0 &&
+                        qmax <= (1ull << 52) && std::isfinite(budget) && used >= 0.0 && used <= budget * (1.0 + 1e-12);
+    if (!(fresh || traced) || maxPh <= 0)
+        return fail(GP_ERR_INVALID, std::string(path) + " has an invalid fixed-point film section");
     std::vector<long long> words(np * 4);
     if (!get(f.get(), words.data(), words.size() * sizeof(long long)))
         return fail(GP_ERR_INVALID, std::string(path) + " is truncated");

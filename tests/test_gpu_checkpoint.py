@@ -2,6 +2,8 @@
 sidecar the reference's readCheckpoint accepts (checked with the oracle's restatement
 of the reader), and a render saved, reloaded and continued is bit-identical to one
 that never stopped and to a single-shot render of all photons."""
+import struct
+
 import numpy as np
 import pytest
 
@@ -62,3 +64,13 @@ def test_load_refuses_mismatch_and_reference_only_files(tmp_path, gplib):
     with pytest.raises(gp.GpError) as ei:
         gp.Film.load_checkpoint(ref_only, guard=77, lib=gplib)
     assert ei.value.status == 2
+    # a fixed-point section with a scale that is not positive, or a saturation value
+    # that is not a power of two, is refused rather than loaded
+    for off, val in ((8, struct.pack("<d", -1.0)), (16, struct.pack("<Q", 3))):
+        bad = bytearray(raw)
+        bad[head + off:head + off + 8] = val
+        p = tmp_path / f"bad{off}.ftbuf"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(gp.GpError) as ei:
+            gp.Film.load_checkpoint(p, guard=77, lib=gplib)
+        assert ei.value.status == 1      # GP_ERR_INVALID
