@@ -363,7 +363,21 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
             const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
             int left = __float_as_int(a.w), right = __float_as_int(b.w);
-            if (ph.hl && ph.hr) {
+            if (!SPILL) {
+                // Branch-free push: the far child is written at sp in every case and
+                // kept (sp advanced) only when both children are hit. Without spill
+                // the tree is shallower than the LDS stack, so sp < SD here.
+                const bool lnear = ph.hl && (!ph.hr || ph.tl <= ph.tr);
+                stk.e2[stk.sp * kTravThreads] =
+                    make_int2(lnear ? right : left, __float_as_int(lnear ? ph.tr : ph.tl));
+                stk.sp += (ph.hl && ph.hr) ? 1 : 0;
+                if (ph.hl || ph.hr) {
+                    node = lnear ? left : right;
+                } else if (!popNear(stk, tmax, node)) {
+                    live = false;
+                    break;
+                }
+            } else if (ph.hl && ph.hr) {
                 bool lfirst = ph.tl <= ph.tr;
                 stk.push(lfirst ? right : left, lfirst ? ph.tr : ph.tl);
                 node = lfirst ? left : right;
@@ -408,7 +422,21 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
             const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
             PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
             int left = __float_as_int(a.w), right = __float_as_int(b.w);
-            if (ph.hl && ph.hr) {
+            if (!SPILL) {
+                // branch-free push, as in traceClosest
+                const bool lnear = ph.hl && (!ph.hr || ph.tl <= ph.tr);
+                stk.e1[stk.sp * kTravThreads] = lnear ? right : left;
+                stk.sp += (ph.hl && ph.hr) ? 1 : 0;
+                if (ph.hl || ph.hr) {
+                    node = lnear ? left : right;
+                } else if (stk.sp > 0) {
+                    float t;
+                    stk.pop(node, t);
+                } else {
+                    live = false;
+                    break;
+                }
+            } else if (ph.hl && ph.hr) {
                 bool lfirst = ph.tl <= ph.tr;
                 stk.push(lfirst ? right : left, 0.0f);
                 node = lfirst ? left : right;
