@@ -214,6 +214,7 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         std::memcpy(&trin[k].w, &mat, 4);
     };
     DeviceBvh dbvh;
+    std::vector<float> quads;   // four-wide nodes of the host-built tree
     if (builder == GP_BVH_DEVICE_LBVH) {
         // input order; the device sorts, builds and gathers
         float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -235,11 +236,12 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
             return st;
         s->nNodes = dbvh.nPairs;
         s->depth = dbvh.depth;
+        s->stackNeed = dbvh.stackNeed;
         s->buildMs = dbvh.buildMs;
         s->dNodes = dbvh.nodes;
         s->dTris = dbvh.tris;
         s->dTrin = dbvh.trin;
-        s->deviceBytes += (int64_t)(dbvh.nPairs * 64 + (int64_t)n * 64);
+        s->deviceBytes += (int64_t)(dbvh.nPairs * 128 + (int64_t)n * 64);
     } else {
         auto t0 = std::chrono::steady_clock::now();
         BvhResult bvh;
@@ -249,7 +251,6 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
             return fail(GP_ERR_INVALID, ex.what());
         }
         auto t1 = std::chrono::steady_clock::now();
-        s->nNodes = (int64_t)(bvh.pairs.size() / 16);
         s->depth = bvh.depth;
         s->buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
         // a triangle may be referenced from several leaves (one entry per reference)
@@ -263,6 +264,10 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
             std::memcpy(&s->leafTris[9 * k], d->verts + 9 * (size_t)prim, 9 * sizeof(float));
         }
         s->pairs = std::move(bvh.pairs);
+        int depth4 = 0;
+        collapse4(s->pairs, quads, depth4);
+        s->nNodes = (int64_t)(quads.size() / 32);
+        s->stackNeed = 3 * depth4;
     }
 
     std::vector<DevMaterial> mats((size_t)d->n_materials);
@@ -307,7 +312,7 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         return GP_OK;
     };
     if (builder != GP_BVH_DEVICE_LBVH) {
-        if ((st = upload(s->pairs.data(), s->pairs.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) return st;
+        if ((st = upload(quads.data(), quads.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) return st;
         if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) return st;
         if ((st = upload(trin.data(), trin.size() * sizeof(float4), (void**)&s->dTrin)) != GP_OK) return st;
     }
@@ -439,7 +444,7 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     ++E.epoch;
     const int lds = E.tun.lds_stack;
-    const int needSpill = std::max(0, s->depth + 1 - lds);
+    const int needSpill = std::max(0, s->stackNeed + 1 - lds);
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams, s->nMirrors > 0)) != GP_OK) return st;
     E.mirrorActive = s->nMirrors > 0;
     hipStream_t strm = device().stream;
@@ -530,7 +535,7 @@ static gp_status probeCommon(gp_scene* s, int64_t n, int& blocks, DevBuf& spill,
     gp_status st = requireDevice();
     if (st != GP_OK) return st;
     blocks = std::max(1, device().cus * 8);
-    spillDepth = std::max(0, s->depth + 1 - lds);
+    spillDepth = std::max(0, s->stackNeed + 1 - lds);
     if (spillDepth > 0) {
         hipError_t e = hipMalloc(&spill.p, (size_t)blocks * 256 * spillDepth * sizeof(int2));
         if (e != hipSuccess) return fail(GP_ERR_OOM, "probe spill allocation failed");

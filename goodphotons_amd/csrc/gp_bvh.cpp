@@ -234,4 +234,52 @@ void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out) {
     (void)stack;
 }
 
+void collapse4(const std::vector<float>& pairs, std::vector<float>& quads, int& depth4) {
+    struct Slot { float lo[3], hi[3]; int link; };
+    auto child = [&](int p, int side) {
+        Slot c;
+        const float* q = &pairs[16 * (size_t)p + (side ? 8 : 0)];
+        for (int a = 0; a < 3; ++a) { c.lo[a] = q[a]; c.hi[a] = q[4 + a]; }
+        std::memcpy(&c.link, &pairs[16 * (size_t)p + (side ? 7 : 3)], 4);
+        return c;
+    };
+    struct Todo { int pair, quad, level; };
+    quads.assign(32, 0.0f);
+    depth4 = 0;
+    std::vector<Todo> todo{{0, 0, 1}};
+    while (!todo.empty()) {
+        const Todo t = todo.back();
+        todo.pop_back();
+        depth4 = std::max(depth4, t.level);
+        Slot s[4];
+        int n = 0;
+        for (int side = 0; side < 2; ++side) {
+            const Slot c = child(t.pair, side);
+            if (c.link < 0) s[n++] = c;
+            else { s[n++] = child(c.link, 0); s[n++] = child(c.link, 1); }
+        }
+        int link[4];
+        for (int k = 0; k < n; ++k) {
+            link[k] = s[k].link;
+            if (link[k] >= 0) {
+                const int nq = (int)(quads.size() / 32);
+                quads.resize(quads.size() + 32, 0.0f);
+                todo.push_back({link[k], nq, t.level + 1});
+                link[k] = nq;
+            }
+        }
+        float* q = &quads[32 * (size_t)t.quad];
+        for (int k = 0; k < 4; ++k) {
+            if (k < n) {
+                for (int a = 0; a < 3; ++a) { q[8 * a + k] = s[k].lo[a]; q[8 * a + 4 + k] = s[k].hi[a]; }
+                std::memcpy(&q[24 + k], &link[k], 4);
+            } else {
+                // empty slot: a point box far outside any scene, linked to slot 0 (valid)
+                for (int a = 0; a < 3; ++a) { q[8 * a + k] = 1e30f; q[8 * a + 4 + k] = 1e30f; }
+                std::memcpy(&q[24 + k], &link[0], 4);
+            }
+        }
+    }
+}
+
 }  // namespace gp

@@ -1,6 +1,6 @@
 // Device BVH builder: a linear BVH (Karras, "Maximizing parallelism in the
-// construction of BVHs, octrees, and k-d trees", HPG 2012) emitted directly in the
-// child-pair layout the traversal reads.
+// construction of BVHs, octrees, and k-d trees", HPG 2012) emitted in the child-pair
+// layout and collapsed into the four-wide nodes the traversal reads.
 //
 //   1. Morton code (21 bits per axis) of each triangle's box centre in the scene's
 //      centre bounds; sort (code, index) pairs (rocPRIM radix sort).
@@ -12,6 +12,9 @@
 //   4. Every internal node whose range holds more than kLeafMax primitives becomes a
 //      pair node; a child with at most kLeafMax primitives becomes a leaf over its
 //      contiguous sorted range. Triangles are gathered into sorted (leaf) order.
+//   5. Pair node i becomes four-wide node i, whose slots are its grandchildren (a
+//      leaf child keeps its own slot), so links keep their indices; the root is
+//      node 0, and nodes at odd pair levels are built but never reached.
 //
 // Boxes are padded and rounded outward exactly like the host builder's (1e-6 pad),
 // so traversal stays conservative; closest-hit results do not depend on the tree.
@@ -212,6 +215,51 @@ __global__ void k_emit_pairs(const KNode* __restrict__ nodes, int n, const float
     q[3] = make_float4(br[3], br[4], br[5], 0.0f);
 }
 
+// Four-wide node i from pair node i (see step 5; the layout is collapse4's in
+// gp_bvh.hpp). `nodes` is null for the single pair of a lone leaf.
+__global__ void k_collapse4(const KNode* __restrict__ nodes, int nInner, const float4* __restrict__ pairs,
+                            float4* __restrict__ quads) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nInner) return;
+    if (nodes) {
+        const KNode nd = nodes[i];
+        if (nd.last - nd.first + 1 <= kLeafMax) return;   // no pair node here
+    }
+    float lo[3][4], hi[3][4];
+    int link[4];
+    int n = 0;
+    for (int side = 0; side < 2; ++side) {
+        const float4* p = pairs + 4 * (size_t)i;
+        const float4 a = side ? p[2] : p[0], b = side ? p[3] : p[1];
+        const int c = __float_as_int(side ? p[1].w : p[0].w);
+        if (c < 0) {
+            lo[0][n] = a.x; lo[1][n] = a.y; lo[2][n] = a.z;
+            hi[0][n] = b.x; hi[1][n] = b.y; hi[2][n] = b.z;
+            link[n++] = c;
+        } else {
+            const float4* q = pairs + 4 * (size_t)c;
+            for (int g = 0; g < 2; ++g) {
+                const float4 ga = g ? q[2] : q[0], gb = g ? q[3] : q[1];
+                lo[0][n] = ga.x; lo[1][n] = ga.y; lo[2][n] = ga.z;
+                hi[0][n] = gb.x; hi[1][n] = gb.y; hi[2][n] = gb.z;
+                link[n++] = __float_as_int(g ? q[1].w : q[0].w);
+            }
+        }
+    }
+    for (int k = n; k < 4; ++k) {   // unused slot: far point box, linked to slot 0
+        for (int a = 0; a < 3; ++a) { lo[a][k] = 1e30f; hi[a][k] = 1e30f; }
+        link[k] = link[0];
+    }
+    float4* o = quads + 8 * (size_t)i;
+    for (int a = 0; a < 3; ++a) {
+        o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    o[6] = make_float4(__int_as_float(link[0]), __int_as_float(link[1]), __int_as_float(link[2]),
+                       __int_as_float(link[3]));
+    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // a scene of <= kLeafMax triangles: one leaf, tested through a pair with itself
 __global__ void k_lone_leaf(const float* __restrict__ leafBox, int n, float4* __restrict__ pairs) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -360,9 +408,17 @@ gp_status buildBvhDevice(const float4* dTrisIn, const float4* dTrinIn, int64_t n
         bool keep = false;
         ~PairsGuard() { if (!keep && p) hipFree(p); }
     } pg{pairs};
+    float4* quads = nullptr;
+    {
+        hipError_t e = hipMalloc((void**)&quads, nPairs * 8 * sizeof(float4));
+        if (e != hipSuccess) return fail(GP_ERR_OOM, "BVH build: node allocation failed");
+    }
+    PairsGuard qg{quads};
     int height = 1;
     if (n <= kLeafMax) {
         hipLaunchKernelGGL(k_lone_leaf, dim3(1), dim3(64), 0, s, (const float*)leafBox.p, n, pairs);
+        hipLaunchKernelGGL(k_collapse4, dim3(1), dim3(64), 0, s, (const KNode*)nullptr, 1, (const float4*)pairs,
+                           quads);
         GP_HIP(hipGetLastError());
     } else {
         DevBuf nodes, parentInner, parentLeaf, box, heights, arrivals;
@@ -381,6 +437,8 @@ gp_status buildBvhDevice(const float4* dTrisIn, const float4* dTrinIn, int64_t n
                            (float*)box.p, (int*)heights.p, (unsigned*)arrivals.p);
         hipLaunchKernelGGL(k_emit_pairs, dim3(blocks(n - 1)), dim3(256), 0, s, (const KNode*)nodes.p, n,
                            (const float*)leafBox.p, (const float*)box.p, pairs);
+        hipLaunchKernelGGL(k_collapse4, dim3(blocks(n - 1)), dim3(256), 0, s, (const KNode*)nodes.p, n - 1,
+                           (const float4*)pairs, quads);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(&height, heights.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
@@ -402,12 +460,13 @@ gp_status buildBvhDevice(const float4* dTrisIn, const float4* dTrinIn, int64_t n
         hipFree(trin);
         return fail(GP_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(ge != hipSuccess ? ge : se));
     }
-    pg.keep = true;
-    out.nodes = pairs;
+    qg.keep = true;   // the pair nodes are freed on return
+    out.nodes = quads;
     out.tris = tris;
     out.trin = trin;
     out.nPairs = (int64_t)nPairs;
     out.depth = std::max(1, height);
+    out.stackNeed = 3 * ((out.depth + 1) / 2);   // four-wide levels: pair levels 1, 3, 5, ...
     out.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return GP_OK;
 }

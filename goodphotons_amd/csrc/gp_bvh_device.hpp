@@ -1,4 +1,4 @@
-// Device BVH builder (linear BVH in the child-pair layout) and the device-side
+// Device BVH builder (linear BVH, collapsed into four-wide nodes) and the device-side
 // eye-to-scene distance used when the tree never exists on the host.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -10,11 +10,12 @@
 namespace gp {
 
 struct DeviceBvh {
-    float4* nodes = nullptr;   // nPairs x 4 float4
+    float4* nodes = nullptr;   // nPairs four-wide nodes x 8 float4 (node i = pair node i collapsed)
     float4* tris = nullptr;    // 3 float4 per triangle, leaf order
     float4* trin = nullptr;    // normal + material per triangle, leaf order
     int64_t nPairs = 0;
     int depth = 0;             // pair nodes on the longest root-to-leaf path
+    int stackNeed = 0;         // traversal stack bound: 3 per four-wide level
     double buildMs = 0.0;
 };
 

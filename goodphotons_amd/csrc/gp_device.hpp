@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 
 #include "gp_types.hpp"
@@ -237,44 +238,17 @@ __device__ __forceinline__ bool hitTri(const RayPre& r, F3 v0, F3 v1, F3 v2, flo
     return true;
 }
 
-// Slab test of the two child boxes of a node. Returns entry distances; a child is
-// hit when its entry <= exit. Exit distances are widened by 2^-21 relative so
-// rounding can only over-report a box (Ize 2013, robust BVH traversal).
-struct PairHit {
-    float tl, tr;
-    bool hl, hr;
-};
-__device__ __forceinline__ PairHit slabPair(const RayPre& r, float4 a, float4 b, float4 c, float4 d,
-                                            float tmin, float tmax) {
-    const float widen = 1.00000048f;
-    float lx0 = (a.x - r.o.x) * r.inv.x, lx1 = (b.x - r.o.x) * r.inv.x;
-    float ly0 = (a.y - r.o.y) * r.inv.y, ly1 = (b.y - r.o.y) * r.inv.y;
-    float lz0 = (a.z - r.o.z) * r.inv.z, lz1 = (b.z - r.o.z) * r.inv.z;
-    float rx0 = (c.x - r.o.x) * r.inv.x, rx1 = (d.x - r.o.x) * r.inv.x;
-    float ry0 = (c.y - r.o.y) * r.inv.y, ry1 = (d.y - r.o.y) * r.inv.y;
-    float rz0 = (c.z - r.o.z) * r.inv.z, rz1 = (d.z - r.o.z) * r.inv.z;
-    float lin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
-    float lout = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax)) * widen;
-    float rin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
-    float rout = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax)) * widen;
-    PairHit p;
-    p.tl = lin;
-    p.tr = rin;
-    p.hl = lin <= lout;
-    p.hr = rin <= rout;
-    return p;
-}
-
 // Traversal kernels run workgroups of this many threads; the LDS stack is laid out
 // with this stride so lane addresses need no multiply.
 constexpr int kTravThreads = 256;
 
 // Traversal stack: the top `SD` entries live in LDS at [slot * 256 + lane], deeper
-// entries spill to a per-thread global area. The host guarantees SD + spill depth
-// >= the tree's depth, and each level pushes at most one entry, so it never
-// overflows. With WITH_T an entry is (node, entry distance) in one 64-bit LDS word;
-// without (any-hit) just the node. SPILL = false is the variant for trees that fit
-// in LDS (no spill area allocated, no spill branches).
+// entries spill to a per-thread global area. The stack holds at most three entries
+// per level of the four-wide tree (the hit siblings of the path's nodes), so a tree
+// of depth D needs 3 D entries; the host guarantees SD + spill depth >= 3 D, so it
+// never overflows. With WITH_T an entry is (node, entry distance) in one 64-bit LDS
+// word; without (any-hit) just the node. SPILL = false is the variant for trees that
+// fit in LDS (no spill area allocated, no spill branches).
 template <int SD, bool WITH_T = true, bool SPILL = true>
 struct Stack {
     int2* e2;             // LDS, this lane's column of SD entries (WITH_T)
@@ -329,16 +303,44 @@ __device__ __forceinline__ void loadLeaf(const float4* __restrict__ tris, int fi
     }
 }
 
-// Closest hit. Returns the leaf-order triangle index (or -1) and its distance.
-// Front-to-back: at a node with both children hit, the nearer is visited and the
-// farther pushed with its entry distance; a popped entry farther than the current
-// hit is skipped. Triangles of a leaf are tested in order; a later triangle only
-// replaces the hit if strictly closer.
-//
-// Loop shape (Aila & Laine 2009, "while-while"): the inner loop walks inner nodes and
-// pops in place when a node's children are both missed, so a lane stays in the loop
-// with its wave instead of leaving it to pop; it exits at a leaf or an empty stack.
-// The visit order per ray is the same as one visit per iteration.
+// Four-wide node visit: slab test of the four slot boxes, then the entry distances
+// (+inf for a missed slot) sorted nearest first together with their links by a
+// five-exchange network; returns the number of slots hit, which come first. Exit
+// distances are widened by 2^-21 relative so rounding can only over-report a box
+// (Ize 2013, robust BVH traversal); a hit's distance is clamped to FLT_MAX so that
+// hits sort ahead of misses even when a ray parallel to a slab makes it infinite.
+__device__ __forceinline__ int visit4(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
+                                      float t[4], int l[4]) {
+    const float widen = 1.00000048f;
+    const float4 lx = ldg4(nd), hx = ldg4(nd + 1), ly = ldg4(nd + 2), hy = ldg4(nd + 3);
+    const float4 lz = ldg4(nd + 4), hz = ldg4(nd + 5), lk = ldg4(nd + 6);
+    const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+    l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float x0 = (LX[k] - r.o.x) * r.inv.x, x1 = (HX[k] - r.o.x) * r.inv.x;
+        const float y0 = (LY[k] - r.o.y) * r.inv.y, y1 = (HY[k] - r.o.y) * r.inv.y;
+        const float z0 = (LZ[k] - r.o.z) * r.inv.z, z1 = (HZ[k] - r.o.z) * r.inv.z;
+        const float tin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        const float tout = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * widen;
+        const bool h = tin <= tout;
+        t[k] = h ? fminf(tin, FLT_MAX) : __int_as_float(0x7f800000);
+        n += h ? 1 : 0;
+    }
+    auto cs = [&](int a, int b) {
+        const bool sw = t[b] < t[a];
+        const float ta = t[a], tb = t[b];
+        const int la = l[a], lb = l[b];
+        t[a] = sw ? tb : ta; t[b] = sw ? ta : tb;
+        l[a] = sw ? lb : la; l[b] = sw ? la : lb;
+    };
+    cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+    return n;
+}
+
 template <int SD, bool SPILL>
 __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax, int& node) {
     while (stk.sp > 0) {
@@ -349,6 +351,10 @@ __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax,
     return false;
 }
 
+// Closest hit over the four-wide tree. Returns the leaf-order triangle index (or -1)
+// and its distance. Front-to-back: the nearest hit slot is visited next and the
+// others pushed farthest first with their entry distances; a popped entry farther
+// than the current hit is skipped. Loop shape: while-while (Aila & Laine 2009).
 template <int SD, bool SPILL>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
@@ -359,32 +365,23 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
     while (true) {
         bool live = true;
         while (node >= 0) {
-            const float4* nd = nodes + 4 * node;
-            const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
-            PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
-            int left = __float_as_int(a.w), right = __float_as_int(b.w);
+            float t[4];
+            int l[4];
+            const int n = visit4(r, nodes + 8 * node, tmin, tmax, t, l);
             if (!SPILL) {
-                // Branch-free push: the far child is written at sp in every case and
-                // kept (sp advanced) only when both children are hit. Without spill
-                // the tree is shallower than the LDS stack, so sp < SD here.
-                const bool lnear = ph.hl && (!ph.hr || ph.tl <= ph.tr);
-                stk.e2[stk.sp * kTravThreads] =
-                    make_int2(lnear ? right : left, __float_as_int(lnear ? ph.tr : ph.tl));
-                stk.sp += (ph.hl && ph.hr) ? 1 : 0;
-                if (ph.hl || ph.hr) {
-                    node = lnear ? left : right;
-                } else if (!popNear(stk, tmax, node)) {
-                    live = false;
-                    break;
-                }
-            } else if (ph.hl && ph.hr) {
-                bool lfirst = ph.tl <= ph.tr;
-                stk.push(lfirst ? right : left, lfirst ? ph.tr : ph.tl);
-                node = lfirst ? left : right;
-            } else if (ph.hl) {
-                node = left;
-            } else if (ph.hr) {
-                node = right;
+                stk.e2[stk.sp * kTravThreads] = make_int2(l[3], __float_as_int(t[3]));
+                stk.sp += n > 3 ? 1 : 0;
+                stk.e2[stk.sp * kTravThreads] = make_int2(l[2], __float_as_int(t[2]));
+                stk.sp += n > 2 ? 1 : 0;
+                stk.e2[stk.sp * kTravThreads] = make_int2(l[1], __float_as_int(t[1]));
+                stk.sp += n > 1 ? 1 : 0;
+            } else {
+                if (n > 3) stk.push(l[3], t[3]);
+                if (n > 2) stk.push(l[2], t[2]);
+                if (n > 1) stk.push(l[1], t[1]);
+            }
+            if (n > 0) {
+                node = l[0];
             } else if (!popNear(stk, tmax, node)) {
                 live = false;
                 break;
@@ -408,7 +405,7 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
     return best;
 }
 
-// Any hit on [tmin, tmax). Same loop shape as traceClosest.
+// Any hit on [tmin, tmax) over the four-wide tree. Same loop shape.
 template <int SD, bool SPILL>
 __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                                          const float4* __restrict__ tris, const RayPre& r,
@@ -418,35 +415,26 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
     while (true) {
         bool live = true;
         while (node >= 0) {
-            const float4* nd = nodes + 4 * node;
-            const float4 a = ldg4(nd), b = ldg4(nd + 1), c = ldg4(nd + 2), d = ldg4(nd + 3);
-            PairHit ph = slabPair(r, a, b, c, d, tmin, tmax);
-            int left = __float_as_int(a.w), right = __float_as_int(b.w);
+            float t[4];
+            int l[4];
+            const int n = visit4(r, nodes + 8 * node, tmin, tmax, t, l);
             if (!SPILL) {
-                // branch-free push, as in traceClosest
-                const bool lnear = ph.hl && (!ph.hr || ph.tl <= ph.tr);
-                stk.e1[stk.sp * kTravThreads] = lnear ? right : left;
-                stk.sp += (ph.hl && ph.hr) ? 1 : 0;
-                if (ph.hl || ph.hr) {
-                    node = lnear ? left : right;
-                } else if (stk.sp > 0) {
-                    float t;
-                    stk.pop(node, t);
-                } else {
-                    live = false;
-                    break;
-                }
-            } else if (ph.hl && ph.hr) {
-                bool lfirst = ph.tl <= ph.tr;
-                stk.push(lfirst ? right : left, 0.0f);
-                node = lfirst ? left : right;
-            } else if (ph.hl) {
-                node = left;
-            } else if (ph.hr) {
-                node = right;
+                stk.e1[stk.sp * kTravThreads] = l[3];
+                stk.sp += n > 3 ? 1 : 0;
+                stk.e1[stk.sp * kTravThreads] = l[2];
+                stk.sp += n > 2 ? 1 : 0;
+                stk.e1[stk.sp * kTravThreads] = l[1];
+                stk.sp += n > 1 ? 1 : 0;
+            } else {
+                if (n > 3) stk.push(l[3], 0.0f);
+                if (n > 2) stk.push(l[2], 0.0f);
+                if (n > 1) stk.push(l[1], 0.0f);
+            }
+            if (n > 0) {
+                node = l[0];
             } else if (stk.sp > 0) {
-                float t;
-                stk.pop(node, t);
+                float tt;
+                stk.pop(node, tt);
             } else {
                 live = false;
                 break;
@@ -465,8 +453,8 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
         }
         if (hit) return true;
         if (stk.sp == 0) break;
-        float t;
-        stk.pop(node, t);
+        float tt;
+        stk.pop(node, tt);
     }
     return false;
 }

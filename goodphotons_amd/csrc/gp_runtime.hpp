@@ -15,8 +15,9 @@
 
 struct gp_scene {
     int64_t nTris = 0;
-    int64_t nNodes = 0;
-    int depth = 0;
+    int64_t nNodes = 0;                // four-wide nodes on the device
+    int depth = 0;                     // child-pair levels of the built tree
+    int stackNeed = 0;                 // traversal stack entries a ray can need (3 per four-wide level)
     float4* dNodes = nullptr;
     float4* dTris = nullptr;
     float4* dTrin = nullptr;
@@ -33,7 +34,7 @@ struct gp_scene {
     double buildMs = 0.0;
     // host copies for camera-distance bounds (empty when the tree was built on the
     // device: the bound is then computed there and cached per eye position)
-    std::vector<float> pairs;          // 16 floats per pair node
+    std::vector<float> pairs;          // 16 floats per pair node (host copy of the built tree)
     std::vector<float> leafTris;       // 9 floats per triangle, leaf order
     std::vector<std::array<double, 12>> emitQuads;   // origin, u, v, normal
     int builder = GP_BVH_HOST_SAH;

@@ -374,17 +374,17 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     if (E.owner && E.owner != srv) return fail(GP_ERR_STATE, "another server has jobs in flight on this device");
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     gp_status st;
-    int maxDepthScene = 0;
+    int maxStackNeed = 0;
     bool mirrors = false;
     for (auto& kv : srv->jobs) {
         if (kv.second.state == JobState::Finished) continue;   // its scene is no longer used
-        maxDepthScene = std::max(maxDepthScene, kv.second.desc.scene->depth);
+        maxStackNeed = std::max(maxStackNeed, kv.second.desc.scene->stackNeed);
         mirrors = mirrors || kv.second.desc.scene->nMirrors > 0;
     }
     const int lds = E.tun.lds_stack;
     if (srv->engineReady && !srv->inflight.empty()) {
         // a change of engine shape would re-allocate under running chunks
-        const int need = std::max(0, maxDepthScene + 1 - lds);
+        const int need = std::max(0, maxStackNeed + 1 - lds);
         if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0) || (mirrors && !E.q.mv[0]) ||
             mirrors != E.mirrorActive) {
             while (!srv->inflight.empty())
@@ -393,7 +393,7 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
         }
     }
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxDepthScene + 1 - lds), 1, mirrors)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxStackNeed + 1 - lds), 1, mirrors)) != GP_OK) return st;
     E.mirrorActive = mirrors;
     const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg || srv->epoch != E.epoch;
     if (fresh) {

@@ -1,6 +1,6 @@
 // Plain-old-data layouts shared by the host runtime and the CDNA4 kernels.
 // Everything the kernels read lives in HBM in these forms:
-//   BVH        child-pair nodes, 64 B each (both children's boxes + links)
+//   BVH        four-wide nodes, 128 B each (four slot boxes in SoA form + four links)
 //   triangles  3 x float4 per triangle in leaf order (vertices; .w = prim id,
 //              material id), plus 1 x float4 geometric normal (.w = material)
 //   paths      structure-of-arrays queues of float4/uint4 (80 B per path), split

@@ -2,8 +2,9 @@
 emulation of HIP (hipemu.hpp), with AddressSanitizer, into tests/emu/_build/.
 
 The kernel and runtime sources are compiled unchanged from goodphotons_amd/csrc; only
-the device BVH builder (rocPRIM) is replaced by a stub. Used by
-tests/test_emulated_kernels.py to run small traces with every memory access checked.
+rocPRIM's device radix sort (used by the device BVH builder) is replaced by a stable
+host sort (fakehip/rocprim). Used by tests/test_emulated_kernels.py to run small
+traces with every memory access checked.
 """
 from __future__ import annotations
 
@@ -17,8 +18,8 @@ CSRC = os.path.join(ROOT, "goodphotons_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libgp_emu.so")
 CXX = "/opt/rocm/llvm/bin/clang++"
-SOURCES = ["gp_kernels.hip", "gp_image.hip", "gp_runtime.cpp", "gp_api.cpp", "gp_server.cpp",
-           "gp_bvh.cpp", "gp_checkpoint.cpp", "gp_mirror.cpp"]
+SOURCES = ["gp_kernels.hip", "gp_image.hip", "gp_bvh_device.hip", "gp_runtime.cpp", "gp_api.cpp",
+           "gp_server.cpp", "gp_bvh.cpp", "gp_checkpoint.cpp", "gp_mirror.cpp"]
 
 
 def asan_runtime() -> str:
@@ -36,8 +37,7 @@ def build() -> str:
              "-I", os.path.join(HERE, "fakehip"), "-I", HERE, "-I", os.path.join(ROOT, "include"),
              "-I", CSRC, "-Wno-unused-result", "-Wno-ignored-attributes"]
     jobs = [(os.path.join(CSRC, s), os.path.join(OUT, os.path.splitext(s)[0] + ".o"), True) for s in SOURCES]
-    jobs += [(os.path.join(HERE, "hipemu.cpp"), os.path.join(OUT, "hipemu.o"), False),
-             (os.path.join(HERE, "bvh_device_stub.cpp"), os.path.join(OUT, "stub.o"), False)]
+    jobs += [(os.path.join(HERE, "hipemu.cpp"), os.path.join(OUT, "hipemu.o"), False)]
 
     def one(job):
         src, obj, as_cxx = job

@@ -34,6 +34,8 @@ struct dim3 {
     constexpr dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 struct float2 { float x, y; };
+struct float3 { float x, y, z; };
+inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
 struct float4 { float x, y, z, w; };
 struct int2 { int x, y; };
 struct uint4 { unsigned x, y, z, w; };
@@ -73,6 +75,8 @@ inline unsigned emu_mbcnt_hi(unsigned m, unsigned acc) {
 #define __builtin_amdgcn_mbcnt_lo emu_mbcnt_lo
 #define __builtin_amdgcn_mbcnt_hi emu_mbcnt_hi
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __clz(int v) { return v == 0 ? 32 : __builtin_clz((unsigned)v); }
+inline int __clzll(long long v) { return v == 0 ? 64 : __builtin_clzll((unsigned long long)v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 void emu_maybe_yield();   // preemption point: lets another lane run (interleaving)
 inline int __float_as_int(float f) { emu_maybe_yield(); int i; std::memcpy(&i, &f, 4); return i; }

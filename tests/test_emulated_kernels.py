@@ -11,6 +11,9 @@ memory access of the device code is bounds-checked without a GPU.
 - Regression for the splat-table race in k_connect: at 128^2 with a 2^14-path queue on
   a 4-workgroup grid, the old table re-insertion (no barrier after it) indexed the LDS
   table with -1 and sent a splat to a garbage film address here; it must run clean.
+- The device BVH builder (linear BVH and its collapse into four-wide nodes) with the
+  four-wide traversal: closest-hit prim ids against the oracle on edge-case scenes and
+  a soup, and c0 films identical with the host-built and the device-built tree.
 """
 import json
 import os
@@ -76,3 +79,13 @@ def test_splat_table_flush_under_load(emulated, orc):
     assert abs(st["splats"] / cst["splats"] - 1) < 1e-3
     m = image_metrics(x, cx, 1 << 14, 1 << 14, res=(128, 128))
     assert m["rel_rmse_max"] < 1e-3
+
+
+def test_device_bvh_builder_and_wide_traversal(emulated):
+    env = dict(os.environ)
+    env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0", "EMU_CUS": "2"})
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_lbvh.py")],
+                       capture_output=True, text=True, env=env, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["c0_films_identical"] and len(out) == 5
