@@ -244,7 +244,8 @@ def main():
                                "UV sphere + SF10 prism, 0.04x0.04 quad light 6500K, 512x512 pinhole, "
                                "2^26 photons/step, depth 16, hero C=4, int64 XYZ film",
                    "photons_per_step": N, "tris": info["n_tris"], "bvh_depth": info["bvh_depth"],
-                   "segments_per_step": segs / args.steps, "device": dev["name"],
+                   "segments_per_step": segs / args.steps,
+                   "device": f"{dev['name'] or 'AMD GPU'} ({dev['arch']}, {dev['compute_units']} CUs)",
                    "tuning": {k: getattr(tun, k) for k, _ in tun._fields_},
                    "matched_rmse": "GPU and CPU trace the same photons (same per-photon PCG "
                                    "streams), so equal photon count is matched RMSE"},
