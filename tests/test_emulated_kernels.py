@@ -7,7 +7,8 @@ memory access of the device code is bounds-checked without a GPU.
   counts and the same film up to float rounding.
 - The flat-mirror connection (k_mirror, admitted here with GP_EXPERIMENTAL_MIRROR=1)
   with interleaved lanes: the configuration in which its old shared LDS stack made
-  lanes pop garbage nodes; it must trace the oracle's photons.
+  lanes pop garbage nodes; it must trace the oracle's photons, with the traversal
+  stack in LDS and with an 8-entry LDS stack that spills to global memory.
 - Regression for the splat-table race in k_connect: at 128^2 with a 2^14-path queue on
   a 4-workgroup grid, the old table re-insertion (no barrier after it) indexed the LDS
   table with -1 and sent a splat to a garbage film address here; it must run clean.
@@ -37,14 +38,15 @@ def emulated():
     return emu.build()
 
 
-def _run(name, cap, n, res, cus, preempt=0, mirror=False):
+def _run(name, cap, n, res, cus, preempt=0, mirror=False, lds=16):
     env = dict(os.environ)
     env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0",
                 "EMU_CUS": str(cus), "EMU_PREEMPT": str(preempt)})
     if mirror:
         env["GP_EXPERIMENTAL_MIRROR"] = "1"
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_trace.py"), name,
-                        str(cap), str(n), str(res)], capture_output=True, text=True, env=env, timeout=900)
+                        str(cap), str(n), str(res), str(lds)], capture_output=True, text=True, env=env,
+                       timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
     return np.array(out["xyz"]), np.array(out["hits"]), out["energy"], out["stats"]
@@ -64,8 +66,9 @@ def test_interleaved_lanes_trace_the_oracles_photons(emulated, orc):
     assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
 
 
-def test_mirror_connection_with_interleaved_lanes(emulated, orc):
-    x, h, e, st = _run("mirror", 4096, 2048, 32, cus=2, preempt=3, mirror=True)
+@pytest.mark.parametrize("lds", [16, 8], ids=["lds_stack", "spilling_stack"])
+def test_mirror_connection_with_interleaved_lanes(emulated, orc, lds):
+    x, h, e, st = _run("mirror", 4096, 2048, 32, cus=2, preempt=3, mirror=True, lds=lds)
     cx, ch, ce, cst = _oracle(orc, "mirror", 2048, 32)
     assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
     m = image_metrics(x, cx, 2048, 2048, res=(32, 32))
