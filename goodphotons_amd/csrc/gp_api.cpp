@@ -444,8 +444,7 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     ++E.epoch;
     const int lds = E.tun.lds_stack;
-    const int needSpill = std::max(0, s->stackNeed + 1 - lds);
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, needSpill, n_cams, s->nMirrors > 0)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, s->stackNeed, n_cams, s->nMirrors > 0)) != GP_OK) return st;
     E.mirrorActive = s->nMirrors > 0;
     hipStream_t strm = device().stream;
 
@@ -535,7 +534,7 @@ static gp_status probeCommon(gp_scene* s, int64_t n, int& blocks, DevBuf& spill,
     gp_status st = requireDevice();
     if (st != GP_OK) return st;
     blocks = std::max(1, device().cus * 8);
-    spillDepth = std::max(0, s->stackNeed + 1 - lds);
+    spillDepth = std::max(0, s->stackNeed - lds);
     if (spillDepth > 0) {
         hipError_t e = hipMalloc(&spill.p, (size_t)blocks * 256 * spillDepth * sizeof(int2));
         if (e != hipSuccess) return fail(GP_ERR_OOM, "probe spill allocation failed");

@@ -1014,7 +1014,7 @@ void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
 #define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
-    GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+    GP_BY_STACK(q.connLds, q.connUseSpill, GP_CALL);
 #undef GP_CALL
 }
 void launchMirror(const Queues& q, const LaunchShape& ls, hipStream_t s) {

@@ -30,6 +30,9 @@ struct Queues {
     int spillDepth;
     int lds;         // LDS stack entries per lane (8, 16, 24 or 32)
     int useSpill;    // 1: the traced scenes are deeper than the LDS stack (spill variant)
+    int connLds;     // k_connect's LDS stack entries (4-byte entries: at least 24 fit at the
+                     // occupancy its registers allow)
+    int connUseSpill;
     unsigned segCap, connSegCap, mirrorSegCap;
 };
 

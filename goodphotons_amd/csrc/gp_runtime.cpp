@@ -409,7 +409,7 @@ gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned lo
     return GP_OK;
 }
 
-gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams, bool mirrors) {
+gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mirrors) {
     if (!tunSet) { tun = defaultTuning(); tunSet = true; }
     DeviceState& d = device();
     const int cus = d.cus > 0 ? d.cus : 256;
@@ -423,13 +423,19 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams, bool mi
     const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
     const bool wantMirror = mirrors || mirrorQueue;
     const bool needMirrorAlloc = wantMirror && (needQueues || !q.mv[0]);
+    // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
+    // spill[sp - SD], and sp < stackNeed). k_connect's stack holds 4-byte entries, so it
+    // keeps at least 24 in LDS at no cost in occupancy (its registers already allow
+    // only four waves per SIMD); the spill area sized for the other kernels covers it.
+    const int needSpill = std::max(0, stackNeed - lds);
+    const int connLds = std::max(lds, 24);
+    const int connUseSpill = stackNeed > connLds ? 1 : 0;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run
-    // when every traced scene fits (needSpill == 0). A change of variant re-captures
-    // the graph.
+    // when every traced scene fits. A change of variant re-captures the graph.
     const int useSpill = needSpill > 0 ? 1 : 0;
     if (!needQueues && !needSpillAlloc && !needMirrorAlloc && q.lds == lds && q.useSpill == useSpill &&
-        graphShape == ls.blocksExtend)
+        q.connLds == connLds && q.connUseSpill == connUseSpill && graphShape == ls.blocksExtend)
         return GP_OK;
     if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
     if (needQueues) {
@@ -505,6 +511,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int needSpill, int cams, bool mi
     q.spillDepth = spillDepth;
     q.lds = lds;
     q.useSpill = useSpill;
+    q.connLds = connLds;
+    q.connUseSpill = connUseSpill;
     graphShape = ls.blocksExtend;
     return GP_OK;
 }

@@ -143,8 +143,9 @@ struct Engine {
     void resetProfile();
     gp_status collectProfile();
 
+    // stackNeed: traversal stack entries the traced scenes can need (gp_scene::stackNeed);
     // mirrors: allocate the mirror-vertex queue (kept from then on, like the camera count)
-    gp_status ensure(unsigned cap, int lds, int spillDepth, int cams = 1, bool mirrors = false);
+    gp_status ensure(unsigned cap, int lds, int stackNeed, int cams = 1, bool mirrors = false);
     void release();
     void releaseQueues();
     void releaseSpill();

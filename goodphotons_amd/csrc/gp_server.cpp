@@ -384,16 +384,17 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     const int lds = E.tun.lds_stack;
     if (srv->engineReady && !srv->inflight.empty()) {
         // a change of engine shape would re-allocate under running chunks
-        const int need = std::max(0, maxStackNeed + 1 - lds);
-        if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0) || (mirrors && !E.q.mv[0]) ||
-            mirrors != E.mirrorActive) {
+        const int need = std::max(0, maxStackNeed - lds);
+        const bool connSpill = maxStackNeed > std::max(lds, 24);
+        if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0) || connSpill != (E.q.connUseSpill != 0) ||
+            (mirrors && !E.q.mv[0]) || mirrors != E.mirrorActive) {
             while (!srv->inflight.empty())
                 if ((st = processOldest(E, srv)) != GP_OK) return st;
             GP_HIP(hipStreamSynchronize(device().stream));
         }
     }
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, std::max(0, maxStackNeed + 1 - lds), 1, mirrors)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, maxStackNeed, 1, mirrors)) != GP_OK) return st;
     E.mirrorActive = mirrors;
     const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg || srv->epoch != E.epoch;
     if (fresh) {
