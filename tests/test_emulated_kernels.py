@@ -33,22 +33,51 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.skipif(not emu.available(), reason="clang++ with AddressSanitizer not found")
 
 
-@pytest.fixture(scope="module")
-def emulated():
-    return emu.build()
-
-
-def _run(name, cap, n, res, cus, preempt=0, mirror=False, lds=16):
+def _env(cus, preempt=0, mirror=False):
     env = dict(os.environ)
     env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0",
                 "EMU_CUS": str(cus), "EMU_PREEMPT": str(preempt)})
     if mirror:
         env["GP_EXPERIMENTAL_MIRROR"] = "1"
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_trace.py"), name,
-                        str(cap), str(n), str(res), str(lds)], capture_output=True, text=True, env=env,
-                       timeout=900)
-    assert r.returncode == 0, r.stderr[-3000:]
-    out = json.loads(r.stdout.strip().splitlines()[-1])
+    return env
+
+
+def _trace(name, cap, n, res, lds=16):
+    return [sys.executable, os.path.join(ROOT, "tests", "emu", "run_trace.py"), name, str(cap), str(n), str(res),
+            str(lds)]
+
+
+# Every emulated run is one mostly single-threaded child process (one lane runs at a
+# time), so the module starts them all at once and each test waits for its own.
+CASES = {
+    "c1_interleaved": (_trace("c1", 4096, 4096, 32), _env(2, preempt=3)),
+    "mirror_lds": (_trace("mirror", 4096, 2048, 32, 16), _env(2, preempt=3, mirror=True)),
+    "mirror_spill": (_trace("mirror", 4096, 2048, 32, 8), _env(2, preempt=3, mirror=True)),
+    "splat_flush": (_trace("c0", 1 << 14, 1 << 14, 128), _env(1)),
+    "device_bvh": ([sys.executable, os.path.join(ROOT, "tests", "emu", "run_lbvh.py")], _env(2)),
+}
+
+
+@pytest.fixture(scope="module")
+def runs():
+    emu.build()
+    procs = {k: subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+             for k, (cmd, env) in CASES.items()}
+    yield procs
+    for p in procs.values():
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+
+
+def _result(runs, key):
+    out, err = runs[key].communicate(timeout=900)
+    assert runs[key].returncode == 0, err[-3000:]
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def _film(runs, key):
+    out = _result(runs, key)
     return np.array(out["xyz"]), np.array(out["hits"]), out["energy"], out["stats"]
 
 
@@ -58,25 +87,25 @@ def _oracle(orc, name, n, res):
     return orc.render(orc.scene(sd), sd.camera, n, hero_c=4, max_bounce=8, threads=8)
 
 
-def test_interleaved_lanes_trace_the_oracles_photons(emulated, orc):
-    x, h, e, st = _run("c1", 4096, 4096, 32, cus=2, preempt=3)
+def test_interleaved_lanes_trace_the_oracles_photons(runs, orc):
+    x, h, e, st = _film(runs, "c1_interleaved")
     cx, ch, ce, cst = _oracle(orc, "c1", 4096, 32)
     assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
     m = image_metrics(x, cx, 4096, 4096, res=(32, 32))
     assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
 
 
-@pytest.mark.parametrize("lds", [16, 8], ids=["lds_stack", "spilling_stack"])
-def test_mirror_connection_with_interleaved_lanes(emulated, orc, lds):
-    x, h, e, st = _run("mirror", 4096, 2048, 32, cus=2, preempt=3, mirror=True, lds=lds)
+@pytest.mark.parametrize("key", ["mirror_lds", "mirror_spill"], ids=["lds_stack", "spilling_stack"])
+def test_mirror_connection_with_interleaved_lanes(runs, orc, key):
+    x, h, e, st = _film(runs, key)
     cx, ch, ce, cst = _oracle(orc, "mirror", 2048, 32)
     assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
     m = image_metrics(x, cx, 2048, 2048, res=(32, 32))
     assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
 
 
-def test_splat_table_flush_under_load(emulated, orc):
-    x, h, e, st = _run("c0", 1 << 14, 1 << 14, 128, cus=1)
+def test_splat_table_flush_under_load(runs, orc):
+    x, h, e, st = _film(runs, "splat_flush")
     cx, ch, ce, cst = _oracle(orc, "c0", 1 << 14, 128)
     assert abs(st["segments"] / cst["segments"] - 1) < 1e-3
     assert abs(st["splats"] / cst["splats"] - 1) < 1e-3
@@ -84,11 +113,6 @@ def test_splat_table_flush_under_load(emulated, orc):
     assert m["rel_rmse_max"] < 1e-3
 
 
-def test_device_bvh_builder_and_wide_traversal(emulated):
-    env = dict(os.environ)
-    env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0", "EMU_CUS": "2"})
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "run_lbvh.py")],
-                       capture_output=True, text=True, env=env, timeout=900)
-    assert r.returncode == 0, r.stderr[-3000:]
-    out = json.loads(r.stdout.strip().splitlines()[-1])
+def test_device_bvh_builder_and_wide_traversal(runs):
+    out = _result(runs, "device_bvh")
     assert out["c0_films_identical"] and len(out) == 5
