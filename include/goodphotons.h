@@ -196,8 +196,8 @@ typedef struct gp_device_info {
 
 typedef struct gp_scene_info {
     int64_t n_tris;
-    int64_t n_nodes;
-    int32_t bvh_depth;       /* longest root-to-leaf path                            */
+    int64_t n_nodes;         /* four-wide nodes the traversal reads                  */
+    int32_t bvh_depth;       /* longest root-to-leaf path of the binary tree built   */
     int32_t n_emitters;
     int64_t device_bytes;    /* BVH + triangles + tables resident in HBM             */
     double build_ms;         /* host BVH build time                                  */
