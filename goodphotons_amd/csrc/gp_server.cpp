@@ -544,8 +544,8 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
     gp_tuning base;
     gp_status st = gp_get_tuning(&base);
     if (st != GP_OK) return st;
-    const int caps[] = {1 << 20, 1 << 21, 1 << 22};
-    const int waves[] = {16, 32};
+    const int caps[] = {1 << 22, 1 << 23, 1 << 24};
+    const int waves[] = {32, 64};
     const int ldss[] = {8, 16};
     double bestRate = -1.0;
     gp_tuning bestT = base;

@@ -21,7 +21,7 @@ from tests.parity import seeded_rays  # noqa: E402
 
 lib = gp.Library(os.path.join(ROOT, "tests", "emu", "_build", "libgp_emu.so"))
 gp._LIB = lib
-lib.set_tuning(queue_capacity=4096)
+lib.set_tuning(queue_capacity=4096, grid_waves_per_cu=16)
 orc = oracle.Oracle()
 out = {}
 for k in (1, 5):
