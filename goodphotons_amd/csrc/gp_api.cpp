@@ -97,7 +97,7 @@ struct DevBuf {
 using namespace gp;
 
 gp_scene::~gp_scene() {
-    void* ptrs[] = {dNodes, dTris, dTrin, dMats, dEms, dCdfs, dEmCdf, dMirrors};
+    void* ptrs[] = {dNodes, dTris, dTrin, dMats, dEms, dCdfs, dEmCdf, dGuides, dMirrors};
     for (void* p : ptrs)
         if (p) hipFree(p);
 }
@@ -319,6 +319,17 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
     if ((st = upload(mats.data(), mats.size() * sizeof(DevMaterial), (void**)&s->dMats)) != GP_OK) return st;
     if ((st = upload(cdfs.data(), cdfs.size() * sizeof(double), (void**)&s->dCdfs)) != GP_OK) return st;
     if ((st = upload(emCdf.data(), emCdf.size() * sizeof(double), (void**)&s->dEmCdf)) != GP_OK) return st;
+    // guide[k] = last bin i in [0, kLambdaBins) with cdf[i] <= k / kCdfGuide
+    std::vector<int32_t> guides((size_t)nEm * (kCdfGuide + 1));
+    for (int i = 0; i < nEm; ++i) {
+        const double* c = &cdfs[(size_t)i * (kLambdaBins + 1)];
+        for (int k = 0; k <= kCdfGuide; ++k) {
+            const double target = (double)k / (double)kCdfGuide;
+            guides[(size_t)i * (kCdfGuide + 1) + (size_t)k] =
+                (int32_t)(std::upper_bound(c, c + kLambdaBins, target) - c) - 1;
+        }
+    }
+    if ((st = upload(guides.data(), guides.size() * sizeof(int32_t), (void**)&s->dGuides)) != GP_OK) return st;
     for (int i = 0; i < nEm; ++i) {
         const gp_emitter& e = d->emitters[i];
         DevEmitter& de = ems[(size_t)i];
@@ -333,6 +344,7 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         de.collimated = e.collimated ? 1 : 0;
         de.power = (float)powers[(size_t)i];
         de.cdf = s->dCdfs + (size_t)i * (kLambdaBins + 1);
+        de.guide = s->dGuides + (size_t)i * (kCdfGuide + 1);
     }
     if ((st = upload(ems.data(), ems.size() * sizeof(DevEmitter), (void**)&s->dEms)) != GP_OK) return st;
     std::vector<DevMirrorPlane> planes;

@@ -127,9 +127,14 @@ __device__ __forceinline__ F3 cie(float w) {
     return F3{X, Y, Z};
 }
 
-// Inversion of a normalised 1-nm CDF over [360, 830), in double as on the host.
-__device__ __forceinline__ double invertCdf(const double* __restrict__ cdf, double u, double& pdf) {
-    int lo = 0, hi = kLambdaBins;
+// Inversion of a normalised 1-nm CDF over [360, 830), in double as on the host: the
+// last bin lo with cdf[lo] <= u. The guide table brackets it (cdf[guide[k]] <= k/G <= u
+// < (k+1)/G < cdf[guide[k+1] + 1] for k = floor(u G), exact since G is a power of two),
+// so the binary search that follows takes ~3 steps instead of 9 and returns the same bin.
+__device__ __forceinline__ double invertCdf(const double* __restrict__ cdf, const int32_t* __restrict__ guide,
+                                            double u, double& pdf) {
+    const int k = min(max((int)(u * (double)kCdfGuide), 0), kCdfGuide - 1);
+    int lo = guide[k], hi = min(guide[k + 1] + 1, kLambdaBins);
     while (lo + 1 < hi) {
         int mid = (lo + hi) >> 1;
         if (cdf[mid] <= u) lo = mid;

@@ -346,13 +346,13 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         float lam[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
         float beta[kMaxHero] = {0.f, 0.f, 0.f, 0.f};
         double pdf0;
-        lam[0] = (float)invertCdf(E.cdf, ub, pdf0);
+        lam[0] = (float)invertCdf(E.cdf, E.guide, ub, pdf0);
         const bool dead = !(pdf0 > 0.0);
         for (int i = 1; i < C; ++i) {
             double uu = ub + (double)i / (double)C;
             if (uu >= 1.0) uu -= 1.0;
             double pd;
-            lam[i] = (float)invertCdf(E.cdf, uu, pd);
+            lam[i] = (float)invertCdf(E.cdf, E.guide, uu, pd);
         }
         const float b = J.basePower / (float)C;
         for (int i = 0; i < C; ++i) beta[i] = b;

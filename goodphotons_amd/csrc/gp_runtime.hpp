@@ -25,6 +25,7 @@ struct gp_scene {
     gp::DevEmitter* dEms = nullptr;
     double* dCdfs = nullptr;
     double* dEmCdf = nullptr;
+    int32_t* dGuides = nullptr;        // per emitter kCdfGuide + 1 CDF guide entries
     gp::DevMirrorPlane* dMirrors = nullptr;   // flat mirror planes (camera connections through them)
     int nMirrors = 0;
     int nEm = 0;

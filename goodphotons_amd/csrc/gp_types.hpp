@@ -19,6 +19,7 @@ namespace gp {
 constexpr int kMaxHero = 4;           // wavelengths per path (hero + 3)
 constexpr int kMaxJobs = 256;         // job slots in one wavefront
 constexpr int kLambdaBins = 470;      // 1-nm bins over [360, 830)
+constexpr int kCdfGuide = 64;         // guide-table cells over [0, 1) for the CDF inversion
 constexpr float kRayEps = 1e-6f;      // origin offset and tmin, as the reference CPU path
 constexpr int kLeafMax = 4;           // triangles per BVH leaf
 // Queues are split into kSegs segments, each appended to through its own counter
@@ -55,6 +56,7 @@ struct DevEmitter {
     float power;        // integral(SPD) * area * pi
     int32_t pad;
     const double* cdf;  // kLambdaBins + 1 entries, normalised
+    const int32_t* guide;  // kCdfGuide + 1 entries: guide[k] = last bin with cdf <= k / kCdfGuide
 };
 
 constexpr int kMaxCams = 16;          // cameras one job splats into (shared photon flight)
