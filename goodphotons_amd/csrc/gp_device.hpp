@@ -308,14 +308,11 @@ __device__ __forceinline__ void loadLeaf(const float4* __restrict__ tris, int fi
     }
 }
 
-// Four-wide node visit: slab test of the four slot boxes, then the entry distances
-// (+inf for a missed slot) sorted nearest first together with their links by a
-// five-exchange network; returns the number of slots hit, which come first. Exit
-// distances are widened by 2^-21 relative so rounding can only over-report a box
-// (Ize 2013, robust BVH traversal); a hit's distance is clamped to FLT_MAX so that
-// hits sort ahead of misses even when a ray parallel to a slab makes it infinite.
-__device__ __forceinline__ int visit4(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
-                                      float t[4], int l[4]) {
+// Slab test of the four slot boxes of a four-wide node: entry distances, hit flags and
+// links. Exit distances are widened by 2^-21 relative so rounding can only over-report
+// a box (Ize 2013, robust BVH traversal).
+__device__ __forceinline__ void slab4(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
+                                      float tin[4], bool h[4], int l[4]) {
     const float widen = 1.00000048f;
     const float4 lx = ldg4(nd), hx = ldg4(nd + 1), ly = ldg4(nd + 2), hy = ldg4(nd + 3);
     const float4 lz = ldg4(nd + 4), hz = ldg4(nd + 5), lk = ldg4(nd + 6);
@@ -323,17 +320,31 @@ __device__ __forceinline__ int visit4(const RayPre& r, const float4* __restrict_
     const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
     const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
     l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
-    int n = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float x0 = (LX[k] - r.o.x) * r.inv.x, x1 = (HX[k] - r.o.x) * r.inv.x;
         const float y0 = (LY[k] - r.o.y) * r.inv.y, y1 = (HY[k] - r.o.y) * r.inv.y;
         const float z0 = (LZ[k] - r.o.z) * r.inv.z, z1 = (HZ[k] - r.o.z) * r.inv.z;
-        const float tin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        tin[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
         const float tout = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * widen;
-        const bool h = tin <= tout;
-        t[k] = h ? fminf(tin, FLT_MAX) : __int_as_float(0x7f800000);
-        n += h ? 1 : 0;
+        h[k] = tin[k] <= tout;
+    }
+}
+
+// Closest-hit visit: the entry distances (+inf for a missed slot) sorted nearest first
+// together with their links by a five-exchange network; returns the number of slots
+// hit, which come first. A hit's distance is clamped to FLT_MAX so that hits sort
+// ahead of misses even when a ray parallel to a slab makes it infinite.
+__device__ __forceinline__ int visit4(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
+                                      float t[4], int l[4]) {
+    float tin[4];
+    bool h[4];
+    slab4(r, nd, tmin, tmax, tin, h, l);
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        t[k] = h[k] ? fminf(tin[k], FLT_MAX) : __int_as_float(0x7f800000);
+        n += h[k] ? 1 : 0;
     }
     auto cs = [&](int a, int b) {
         const bool sw = t[b] < t[a];
@@ -351,25 +362,9 @@ __device__ __forceinline__ int visit4(const RayPre& r, const float4* __restrict_
 // sorted visit); returns the hit count.
 __device__ __forceinline__ int visit4Any(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
                                          bool h[4], int l[4]) {
-    const float widen = 1.00000048f;
-    const float4 lx = ldg4(nd), hx = ldg4(nd + 1), ly = ldg4(nd + 2), hy = ldg4(nd + 3);
-    const float4 lz = ldg4(nd + 4), hz = ldg4(nd + 5), lk = ldg4(nd + 6);
-    const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
-    l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
-    int n = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float x0 = (LX[k] - r.o.x) * r.inv.x, x1 = (HX[k] - r.o.x) * r.inv.x;
-        const float y0 = (LY[k] - r.o.y) * r.inv.y, y1 = (HY[k] - r.o.y) * r.inv.y;
-        const float z0 = (LZ[k] - r.o.z) * r.inv.z, z1 = (HZ[k] - r.o.z) * r.inv.z;
-        const float tin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        const float tout = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * widen;
-        h[k] = tin <= tout;
-        n += h[k] ? 1 : 0;
-    }
-    return n;
+    float tin[4];
+    slab4(r, nd, tmin, tmax, tin, h, l);
+    return (h[0] ? 1 : 0) + (h[1] ? 1 : 0) + (h[2] ? 1 : 0) + (h[3] ? 1 : 0);
 }
 
 template <int SD, bool SPILL>
