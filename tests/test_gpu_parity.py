@@ -81,11 +81,12 @@ def _check_closest(gplib, orc, sd, rays, name, max_frac=0.1):
     assert np.all(np.abs(gt[both] - ct[both]) <= 2e-6 * (1.0 + ct[both]))
 
 
-def test_any_hit_matches_oracle(gplib, orc):
+@pytest.mark.parametrize("axis", [False, True], ids=["uniform_dirs", "axis_dirs"])
+def test_any_hit_matches_oracle(gplib, orc, axis):
     sd = scenes.caustics_c1(64)
     gs = gp.GpuScene(sd, gplib)
     cs = orc.scene(sd)
-    rays = seeded_rays(sd, 50000, 99)
+    rays = _axis_rays(sd, 50000, 99) if axis else seeded_rays(sd, 50000, 99)
     rng = np.random.default_rng(5)
     tmax = rng.uniform(0.05, 1.5, len(rays)).astype(np.float32)
     occ_g = gs.any_hit(rays, tmax, tmin=1e-6)
