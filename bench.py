@@ -8,9 +8,10 @@ float32 transport, int64 fixed-point XYZ film. One step = one full render of the
 
 Prints one JSON line. `value` = segments / wall time over the timed steps.
 `roofline` is for the dominant kernel, from HIP events on the engine stream in a
-profiled step of the same workload. `cpu_baseline` times the CPU oracle (the
-repository's restatement, bit-identical to the reference's CPU forward path) on a
-bounded sample of the same workload on this host's cores.
+profiled step of the same workload. `cpu_baseline` times the reference's own CPU
+forward path (oracle/_ref, built from its sources; kind "reference"), or the oracle's
+restatement of it where that build is absent (kind "port"), on a bounded sample of the
+same workload on this host's cores.
 """
 from __future__ import annotations
 
