@@ -1,7 +1,7 @@
 """Full-size cases. The 5,242,880-triangle noise-displaced icosphere: closest-hit prim
 ids at full scale against the oracle's BVH (every mismatch must be an ambiguous ray by
-the oracle's brute-force margin test), and a small-budget film against the oracle on
-the same photons. The bench's c1 workload at full size: size-independent properties
+the oracle's brute-force margin test), a small-budget film against the oracle on the
+same photons, and the full budget's run-to-run and batch-additivity identities. The bench's c1 workload at full size: size-independent properties
 (run-to-run bit identity, additivity of seed-offset batches, ledger closure in
 expectation)."""
 import numpy as np
@@ -62,6 +62,25 @@ def test_c2_film_matches_oracle(c2, orc):
     print(f"[c2] film {m} segments gpu {gst['segments']} cpu {cst['segments']}", flush=True)
     assert m["rel_rmse_max"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
     assert abs(gst["segments"] / cst["segments"] - 1) < 5e-3
+
+
+def test_c2_full_budget_properties(c2):
+    """c2 at its full budget (1024^2, 2^28 photons, depth 12; the deep tree takes the
+    spilling traversal stack): bit-identical run to run, and two seed-offset halves add
+    up to the single shot word for word."""
+    sd, gs, cs = c2
+    cam = sd.with_camera_res(1024).camera
+    n = 1 << 28
+    kw = dict(hero_c=4, max_depth=12, rr_depth=sd.rr_depth)
+    films = []
+    for parts in (1, 1, 2):
+        f = gp.Film(1024, 1024, max_photons=n, lib=gs.L)
+        for k in range(parts):
+            gp.trace(gs, cam, f, n // parts, seed_base=k * (n // parts), **kw)
+        films.append(f.raw()[0])
+        f.close()
+    print(f"[c2 full] lit pixels {int((films[0][:, 1] > 0).sum())}", flush=True)
+    assert np.array_equal(films[0], films[1]) and np.array_equal(films[0], films[2])
 
 
 def test_c1_bench_workload_properties(gplib):
