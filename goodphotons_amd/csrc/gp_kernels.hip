@@ -3,7 +3,7 @@
 // One iteration of the wavefront:
 //   k_schedule  (1 block)  refill free queue slots from the jobs' photon budgets
 //   k_emit                 emission + hero bundle + direct light->camera connection
-//   k_extend               closest hit for every live path (LDS-stack traversal)
+//   k_extend               closest hit for every live path (four-wide nodes, LDS stack)
 //   k_shade                material scatter, Russian roulette, camera connection,
 //                          survivors compacted into the other queue by wave ballot
 //   k_connect              any-hit for every connection; unoccluded ones splat into
