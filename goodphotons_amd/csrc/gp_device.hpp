@@ -3,7 +3,7 @@
 // Arithmetic follows the reference's CPU forward path in float32, except where a
 // double keeps the same sample: the photon RNG (PCG32 seeded per photon by
 // SplitMix64, src/rng.h:6-42) and the wavelength CDF inversion
-// (src/spectrum.h:374-383) run in integer / double so the GPU draws exactly the
+// (EmissionSampler::sampleAt, src/spectrum.h:270-279) run in integer / double so the GPU draws exactly the
 // same photons as the CPU reference for the same (seedBase, salt).
 #pragma once
 #include <hip/hip_runtime.h>

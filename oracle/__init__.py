@@ -280,6 +280,8 @@ class Oracle(_CpuRenderer):
                                 C.POINTER(abi.Energy), _u64p]
         L.or_closest_hit_margin.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_double, C.c_double,
                                             _i32p, _dp, _dp, _u8p]
+        L.or_closest_hit_ambiguity.argtypes = [C.c_void_p, _dp, _i32p, C.c_int64, C.c_double,
+                                               C.c_double, _i32p, _dp, _dp, _u8p, _dp, _dp, _u8p]
 
     def render(self, scene: CpuScene, cam, n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4,
                max_bounce=32, rr_depth=0, threads=1):
@@ -303,6 +305,25 @@ class Oracle(_CpuRenderer):
         self.lib.or_closest_hit_margin(scene.h, _ptr(rays, _dp), n, tmin, rel_margin,
                                        _ptr(prim, _i32p), _ptr(t, _dp), _ptr(t2, _dp), _ptr(edge, _u8p))
         return prim, t, t2, edge.astype(bool)
+
+    def closest_hit_ambiguity(self, scene: CpuScene, rays, query, tmin=1e-6, rel_margin=1e-5):
+        """Brute-force ambiguity report per ray (or_closest_hit_ambiguity): the exact
+        closest hit, the nearest grown-hit distance of another triangle, whether the
+        winner is grazed, the nearest robust hit, and the grown-hit distance and graze
+        flag of the queried triangle query[i] (-1 = none)."""
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        q = np.ascontiguousarray(query, np.int32)
+        n = len(rays)
+        assert len(q) == n
+        prim = np.zeros(n, np.int32)
+        t, t2, trob, tq = (np.zeros(n) for _ in range(4))
+        edge = np.zeros(n, np.uint8)
+        qg = np.zeros(n, np.uint8)
+        self.lib.or_closest_hit_ambiguity(scene.h, _ptr(rays, _dp), _ptr(q, _i32p), n, tmin, rel_margin,
+                                          _ptr(prim, _i32p), _ptr(t, _dp), _ptr(t2, _dp), _ptr(edge, _u8p),
+                                          _ptr(trob, _dp), _ptr(tq, _dp), _ptr(qg, _u8p))
+        return {"prim": prim, "t": t, "t2": t2, "edge": edge.astype(bool), "t_robust": trob, "tq": tq,
+                "q_graze": qg.astype(bool)}
 
 
 class Reference(_CpuRenderer):

@@ -7,9 +7,9 @@
 // order changes bits (vector ops are component-wise left to right, divisions stay
 // divisions), because the films are compared with the reference's bit for bit.
 // Followed files (reference repository, src/):
-//   rng.h:6-53 (Pcg32, mix64, seedUnit, cosineHemisphere), linalg.h:5-39,
-//   color.h:8-39 (CIE fit, cieYIntegral), spectrum.h:185-248, 352-389 (Planck,
-//   Sellmeier, Cauchy, EmissionSampler), hero.h:157-179 (sampleBundle, maxOf),
+//   rng.h:6-52 (Pcg32, mix64, seedUnit, cosineHemisphere), linalg.h:5-39,
+//   color.h:8-39 (CIE fit, cieYIntegral), spectrum.h:80-143, 247-284 (Planck,
+//   Sellmeier, Cauchy, EmissionSampler), hero.h:106-127 (sampleBundle, maxOf),
 //   geometry.h:51-99 (Tri::finalize normal), :174-242 (orientedGeoN, adjoint,
 //   terminator), :256-369 (watertight triangle test), bvh.h:18-250 (Aabb, binned
 //   SAH build, closest/any traversal), scene.h:1306, 1491-1528, 1851-1857, 1907-1944,
@@ -1080,57 +1080,104 @@ int or_closest_hit_linear(void* sp, const double* rays, int64_t n, double tmin, 
     return 0;
 }
 
+// The triangle test with a margin (an extension of this repository, no reference
+// counterpart): the edge functions of the sheared frame (the watertight test's U, V,
+// W) compared against +-relMargin times their magnitude scale
+//   scale = (|Cx|+|Cy|)(|Bx|+|By|) + (|Ax|+|Ay|)(|Cx|+|Cy|) + (|Bx|+|By|)(|Ax|+|Ay|),
+// which bounds |U| + |V| + |W|. `grown` = hit with every edge moved outwards by the
+// margin, `shrunk` = hit with every edge moved inwards (a robust hit); tt = the
+// plane distance. A triangle that is grown- but not shrunk-hit is grazed by the ray.
+struct MarginHit {
+    bool grown = false, shrunk = false;
+    double tt = INFINITY;
+};
+MarginHit marginTest(const Shear& sh, const V3& o, const Tri& tri, double tmin, double relMargin) {
+    MarginHit r;
+    const int kx = sh.kx, ky = sh.ky, kz = sh.kz;
+    V3 A = tri.v0 - o, B = tri.v1 - o, C = tri.v2 - o;
+    double Ax = at(A, kx) - sh.Sx * at(A, kz), Ay = at(A, ky) - sh.Sy * at(A, kz);
+    double Bx = at(B, kx) - sh.Sx * at(B, kz), By = at(B, ky) - sh.Sy * at(B, kz);
+    double Cx = at(C, kx) - sh.Sx * at(C, kz), Cy = at(C, ky) - sh.Sy * at(C, kz);
+    double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    double scale = (std::fabs(Cx) + std::fabs(Cy)) * (std::fabs(Bx) + std::fabs(By)) +
+                   (std::fabs(Ax) + std::fabs(Ay)) * (std::fabs(Cx) + std::fabs(Cy)) +
+                   (std::fabs(Bx) + std::fabs(By)) * (std::fabs(Ax) + std::fabs(Ay));
+    double mg = relMargin * scale;
+    double det = U + V + W;
+    if (det == 0.0) return r;
+    double sgn = det > 0 ? 1.0 : -1.0;
+    if (!((sgn * U >= -mg) && (sgn * V >= -mg) && (sgn * W >= -mg))) return r;
+    double T = U * (sh.Sz * at(A, kz)) + V * (sh.Sz * at(B, kz)) + W * (sh.Sz * at(C, kz));
+    double tt = T / det;
+    if (tt < tmin * (1.0 - relMargin) - relMargin) return r;
+    r.grown = true;
+    r.tt = tt;
+    r.shrunk = (sgn * U >= mg) && (sgn * V >= mg) && (sgn * W >= mg) && tt >= tmin;
+    return r;
+}
+
 // Ambiguity report for the closest-hit parity probe (an extension of this
-// repository, no reference counterpart). For each ray: the exact closest hit, the
-// nearest hit distance t2 of any OTHER triangle when every triangle is grown by a
-// relative margin on its scaled edge functions, and an `edge` flag set when the
-// winning triangle is hit with the margin but missed when shrunk by it (the ray
-// grazes its boundary). A float32 closest hit may legitimately differ from the
-// double one only when t2 is within a relative tie tolerance of t or `edge` is set.
-// Brute force over all triangles; for tests on small and medium scenes.
-int or_closest_hit_margin(void* sp, const double* rays, int64_t n, double tmin, double relMargin,
-                          int32_t* prim, double* t, double* t2, uint8_t* edge) {
+// repository, no reference counterpart). For each ray, brute force over all
+// triangles:
+//   prim, t   the exact closest hit (the watertight test, as or_closest_hit_linear);
+//   t2        the nearest grown-hit distance of any OTHER triangle than `prim`;
+//   edge      `prim` is grown- but not shrunk-hit (the ray grazes its boundary);
+//   tRobust   the nearest shrunk-hit distance of any triangle (inf if none);
+//   tq, qGraze  for the queried triangle query[i] (the other side's answer; -1 =
+//             none): its grown-hit distance (inf if not grown-hit) and whether it
+//             is grazed.
+// Every output pointer after `edge` may be null. A float32 closest hit may differ
+// from this double one only by a near-tie (t2 within a relative tie tolerance of t)
+// or by a grazed triangle; the tests classify each mismatch from these fields.
+int or_closest_hit_ambiguity(void* sp, const double* rays, const int32_t* query, int64_t n,
+                             double tmin, double relMargin, int32_t* prim, double* t, double* t2,
+                             uint8_t* edge, double* tRobust, double* tq, uint8_t* qGraze) {
     const Scene& S = *static_cast<Scene*>(sp);
-    for (int64_t i = 0; i < n; ++i) {
+    auto one = [&](int64_t i) {
         V3 o = v3(rays + 6 * i), d = v3(rays + 6 * i + 3);
         Hit h;
         const Shear sh = makeShear(d);
         for (size_t k = 0; k < S.tris.size(); ++k) hitTriangle(sh, o, d, S.tris[k], (int)k, tmin, h);
         prim[i] = h.valid ? h.prim : -1;
         t[i] = h.valid ? h.t : INFINITY;
-        double best2 = INFINITY;
-        bool grazing = false;
+        const int q = query ? query[i] : -1;
+        double best2 = INFINITY, robust = INFINITY, qt = INFINITY;
+        bool grazing = false, qg = false;
         for (size_t k = 0; k < S.tris.size(); ++k) {
-            const Tri& tri = S.tris[k];
-            const int kx = sh.kx, ky = sh.ky, kz = sh.kz;
-            V3 A = tri.v0 - o, B = tri.v1 - o, C = tri.v2 - o;
-            double Ax = at(A, kx) - sh.Sx * at(A, kz), Ay = at(A, ky) - sh.Sy * at(A, kz);
-            double Bx = at(B, kx) - sh.Sx * at(B, kz), By = at(B, ky) - sh.Sy * at(B, kz);
-            double Cx = at(C, kx) - sh.Sx * at(C, kz), Cy = at(C, ky) - sh.Sy * at(C, kz);
-            double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
-            double scale = (std::fabs(Cx) + std::fabs(Cy)) * (std::fabs(Bx) + std::fabs(By)) +
-                           (std::fabs(Ax) + std::fabs(Ay)) * (std::fabs(Cx) + std::fabs(Cy)) +
-                           (std::fabs(Bx) + std::fabs(By)) * (std::fabs(Ax) + std::fabs(Ay));
-            double mg = relMargin * scale;
-            double det = U + V + W;
-            if (det == 0.0) continue;
-            double sgn = det > 0 ? 1.0 : -1.0;
-            bool grown = (sgn * U >= -mg) && (sgn * V >= -mg) && (sgn * W >= -mg);
-            if (!grown) continue;
-            double T = U * (sh.Sz * at(A, kz)) + V * (sh.Sz * at(B, kz)) + W * (sh.Sz * at(C, kz));
-            double tt = T / det;
-            if (tt < tmin * (1.0 - relMargin) - relMargin) continue;
-            bool shrunk = (sgn * U >= mg) && (sgn * V >= mg) && (sgn * W >= mg);
+            const MarginHit m = marginTest(sh, o, S.tris[k], tmin, relMargin);
+            if (!m.grown) continue;
+            if (m.shrunk && m.tt < robust) robust = m.tt;
+            if ((int)k == q) {
+                qt = m.tt;
+                qg = !m.shrunk;
+            }
             if ((int)k == h.prim) {
-                if (!shrunk) grazing = true;
+                if (!m.shrunk) grazing = true;
                 continue;
             }
-            if (tt < best2) best2 = tt;
+            if (m.tt < best2) best2 = m.tt;
         }
         t2[i] = best2;
         edge[i] = grazing ? 1 : 0;
-    }
+        if (tRobust) tRobust[i] = robust;
+        if (tq) tq[i] = qt;
+        if (qGraze) qGraze[i] = qg ? 1 : 0;
+    };
+    // rays are independent: split them over the host's threads (large scenes)
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(n, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int64_t w = 0; w < nt; ++w)
+        pool.emplace_back([&, w] {
+            for (int64_t i = w; i < n; i += nt) one(i);
+        });
+    for (auto& th : pool) th.join();
     return 0;
+}
+
+int or_closest_hit_margin(void* sp, const double* rays, int64_t n, double tmin, double relMargin,
+                          int32_t* prim, double* t, double* t2, uint8_t* edge) {
+    return or_closest_hit_ambiguity(sp, rays, nullptr, n, tmin, relMargin, prim, t, t2, edge, nullptr,
+                                    nullptr, nullptr);
 }
 
 int or_occluded(void* sp, const double* rays, const double* maxDist, int64_t n, uint8_t* occ) {

@@ -14,7 +14,9 @@
 //     incidence, energy ledger closure, brute-force closest hit).
 //   rr_depth > 0 and the closest-hit ambiguity margin are this repository's own
 //     extensions with no reference counterpart: rr_depth is pinned by unbiasedness
-//     (same expectation as rr_depth = 0) and the ledger; the margin by brute force.
+//     (same expectation as rr_depth = 0) and the ledger; the margin report by
+//     constructed rays whose answers the geometry fixes (exact ties, shared edges,
+//     silhouette offsets on either side of the margin; tests/test_oracle_margin.py).
 #pragma once
 #include <cstdint>
 #include "../include/goodphotons.h"
@@ -38,6 +40,12 @@ int or_closest_hit_linear(void* s, const double* rays, int64_t n, double tmin, i
                           double* t);
 int or_closest_hit_margin(void* s, const double* rays, int64_t n, double tmin, double relMargin,
                           int32_t* prim, double* t, double* t2, uint8_t* edge);
+// Per ray: exact closest hit (prim, t); nearest grown-hit distance of another
+// triangle (t2); winner grazed (edge); nearest robust (shrunk) hit (tRobust); and,
+// for the queried triangle query[i], its grown-hit distance (tq) and graze flag.
+int or_closest_hit_ambiguity(void* s, const double* rays, const int32_t* query, int64_t n,
+                             double tmin, double relMargin, int32_t* prim, double* t, double* t2,
+                             uint8_t* edge, double* tRobust, double* tq, uint8_t* qGraze);
 int or_occluded(void* s, const double* rays, const double* maxDist, int64_t n, uint8_t* occ);
 
 void or_pcg_stream(uint64_t seq, uint64_t seed, int n, uint32_t* out);

@@ -256,7 +256,7 @@ int ref_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double
     EmissionSampler es; es.build(toSpectrum(*s), 1.0);
     return hero::sampleBundle(es, u, C, lam, pdf) ? 1 : 0;
 }
-// n cosine-weighted directions about `nrm` (src/rng.h:45-53), each drawn from a
+// n cosine-weighted directions about `nrm` (src/rng.h:45-52), each drawn from a
 // Pcg32 seeded (seq, seed) in sequence.
 void ref_cosine_hemisphere(const double* nrm, uint64_t seq, uint64_t seed, int n, double* out) {
     Pcg32 r; r.seed(seq, seed);

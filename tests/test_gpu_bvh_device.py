@@ -9,7 +9,7 @@ import pytest
 
 import goodphotons_amd as gp
 from goodphotons_amd import abi, scenes
-from tests.parity import seeded_rays
+from tests.parity import check_prim_parity, seeded_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -20,19 +20,9 @@ def _check_probe(gplib, orc, sd, rays, max_frac=0.1):
     cs = orc.scene(sd)
     gprim, gt = gs.closest_hit(rays, tmin=1e-6)
     cprim, ct = cs.closest_hit(rays.astype(np.float64), tmin=1e-6)
-    diff = np.nonzero(gprim != cprim)[0]
-    amb = np.zeros(len(diff), bool)
-    if len(diff):
-        p, t, t2, edge = orc.closest_hit_margin(cs, rays[diff].astype(np.float64), 1e-6, 1e-5)
-        assert np.array_equal(t, ct[diff])
-        near_tie = np.isfinite(t2) & (np.abs(t2 - t) <= 1e-6 * np.maximum(np.abs(t), 1e-30))
-        amb = near_tie | edge | (~np.isfinite(t) & np.isfinite(t2) & (gprim[diff] >= 0))
-    print(f"[lbvh] {sd.name} tris={sd.n_tris} depth={info['bvh_depth']} build={info['build_ms']:.1f} ms "
-          f"mismatches={len(diff)} ambiguous={int(amb.sum())}", flush=True)
-    assert np.all(amb), f"{int((~amb).sum())} unambiguous prim-id mismatches"
-    assert len(diff) <= len(rays) * max_frac
-    both = (gprim == cprim) & (cprim >= 0)
-    assert np.all(np.abs(gt[both] - ct[both]) <= 2e-6 * (1.0 + ct[both]))
+    print(f"[lbvh] {sd.name} tris={sd.n_tris} depth={info['bvh_depth']} build={info['build_ms']:.1f} ms",
+          flush=True)
+    check_prim_parity(orc, cs, rays, gprim, gt, cprim, ct, "lbvh_" + sd.name, max_tie_frac=max_frac)
     return gs, info
 
 

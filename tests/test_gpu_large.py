@@ -1,6 +1,6 @@
 """Full-size cases. The 5,242,880-triangle noise-displaced icosphere: closest-hit prim
-ids at full scale against the oracle's BVH (every mismatch must be an ambiguous ray by
-the oracle's brute-force margin test), a small-budget film against the oracle on the
+ids at full scale against the oracle's BVH (mismatches only on near-ties, plus bounded
+boundary grazes, by the oracle's brute-force ambiguity report), a small-budget film against the oracle on the
 same photons, and the full budget's run-to-run and batch-additivity identities. The bench's c1 workload at full size: size-independent properties
 (run-to-run bit identity, additivity of seed-offset batches, ledger closure in
 expectation)."""
@@ -9,7 +9,7 @@ import pytest
 
 import goodphotons_amd as gp
 from goodphotons_amd import scenes
-from tests.parity import image_metrics, seeded_rays
+from tests.parity import check_prim_parity, image_metrics, seeded_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -37,16 +37,8 @@ def test_c2_prim_ids_full_scale(c2, orc):
     rays = np.concatenate([o, d], 1).astype(np.float32)
     gprim, gt = gs.closest_hit(rays)
     cprim, ct = cs.closest_hit(rays.astype(np.float64))
-    diff = np.nonzero(gprim != cprim)[0]
-    print(f"[c2] hits {int((cprim >= 0).sum())} mismatches {len(diff)}", flush=True)
-    assert len(diff) <= n // 1000
-    if len(diff):
-        p, t, t2, edge = orc.closest_hit_margin(cs, rays[diff].astype(np.float64), 1e-6, 1e-5)
-        near_tie = np.isfinite(t2) & (np.abs(t2 - t) <= 1e-6 * np.maximum(np.abs(t), 1e-30))
-        amb = near_tie | edge | (~np.isfinite(t) & np.isfinite(t2) & (gprim[diff] >= 0))
-        assert np.all(amb), f"{int((~amb).sum())} unambiguous mismatches"
-    both = (gprim == cprim) & (cprim >= 0)
-    assert np.all(np.abs(gt[both] - ct[both]) <= 2e-6 * (1.0 + ct[both]))
+    print(f"[c2] hits {int((cprim >= 0).sum())} mismatches {int((gprim != cprim).sum())}", flush=True)
+    check_prim_parity(orc, cs, rays, gprim, gt, cprim, ct, "c2_full", max_tie_frac=1e-3)
 
 
 def test_c2_film_matches_oracle(c2, orc):
