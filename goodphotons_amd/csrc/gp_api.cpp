@@ -19,7 +19,7 @@ const char* lastError();
 namespace {
 
 // Host evaluation of a descriptor spectrum in double: the same closed forms the
-// reference's Spectrum closures compute (src/spectrum.h:185-248), used for the
+// reference's Spectrum closures compute (src/spectrum.h:80-143), used for the
 // emitter wavelength CDFs.
 double specEvalHost(const gp_spectrum& s, double w) {
     if (s.band_hi > s.band_lo && !(w >= s.band_lo && w <= s.band_hi)) return 0.0;
@@ -51,7 +51,7 @@ double specEvalHost(const gp_spectrum& s, double w) {
     }
 }
 
-// 1-nm emission CDF over [360, 830) (EmissionSampler::build, src/spectrum.h:357-369).
+// 1-nm emission CDF over [360, 830) (EmissionSampler::build, src/spectrum.h:253-265).
 double buildCdf(const gp_spectrum& spd, double* cdf) {
     double acc = 0.0;
     cdf[0] = 0.0;
@@ -146,13 +146,6 @@ gp_status gp_device_query(gp_device_info* out) {
 }
 
 // ------------------------------------------------------------------ scenes
-// Mirror materials stay rejected until the flat-mirror connection has run on the GPU;
-// GP_EXPERIMENTAL_MIRROR=1 in the environment admits them (CPU checks of k_mirror).
-static bool experimentalMirror() {
-    const char* v = std::getenv("GP_EXPERIMENTAL_MIRROR");
-    return v && v[0] == '1' && v[1] == 0;
-}
-
 static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out) {
     if (!d || !out) return fail(GP_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -162,10 +155,6 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
     if (d->n_tris >= (int64_t(1) << 28)) return fail(GP_ERR_CAPACITY, "more than 2^28 triangles");
     for (int i = 0; i < d->n_materials; ++i) {
         const gp_material& m = d->materials[i];
-        if (m.type == GP_MAT_MIRROR && !experimentalMirror())
-            return fail(GP_ERR_UNSUPPORTED,
-                        "Mirror material: the flat-mirror camera connection is switched off until its "
-                        "fix (separate LDS stacks for its two traversals) is verified on the GPU");
         if (m.type == GP_MAT_GLOSSY && !(m.roughness > 0.0))
             return fail(GP_ERR_UNSUPPORTED, "Glossy with roughness <= 0 is a planar mirror");
         if (m.type != GP_MAT_DIFFUSE && m.type != GP_MAT_DIELECTRIC && m.type != GP_MAT_MIRROR &&

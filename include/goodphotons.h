@@ -110,9 +110,7 @@ enum {
  * coplanar Mirror triangles of a plane (at most 64 planes, gp_mirror_planes). One
  * specular vertex per connection (a mirror seen in a mirror stays dark), pinhole
  * camera only. Glossy with roughness <= 0 and the reference's HalfMirror are not
- * supported. In this release gp_scene_create still rejects Mirror with
- * GP_ERR_UNSUPPORTED unless GP_EXPERIMENTAL_MIRROR=1 is set in the environment: the
- * connection is built and checked on the CPU but not yet verified on the device. */
+ * supported. */
 typedef struct gp_material {
     int32_t type;
     int32_t reserved;

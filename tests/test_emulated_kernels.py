@@ -5,10 +5,10 @@ memory access of the device code is bounds-checked without a GPU.
 - A small trace with lanes preempted inside their loops (so lanes of a block interleave
   as waves do) traces exactly the oracle's photons: equal segment, connection and splat
   counts and the same film up to float rounding.
-- The flat-mirror connection (k_mirror, admitted here with GP_EXPERIMENTAL_MIRROR=1)
-  with interleaved lanes: the configuration in which its old shared LDS stack made
-  lanes pop garbage nodes; it must trace the oracle's photons, with the traversal
-  stack in LDS and with an 8-entry LDS stack that spills to global memory.
+- The flat-mirror connection (k_mirror) with interleaved lanes: the configuration in
+  which its old shared LDS stack made lanes pop garbage nodes; it must trace the
+  oracle's photons, with the traversal stack in LDS and with an 8-entry LDS stack that
+  spills to global memory.
 - Regression for the splat-table race in k_connect: at 128^2 with a 2^14-path queue on
   a 4-workgroup grid, the old table re-insertion (no barrier after it) indexed the LDS
   table with -1 and sent a splat to a garbage film address here; it must run clean.
@@ -33,12 +33,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.skipif(not emu.available(), reason="clang++ with AddressSanitizer not found")
 
 
-def _env(cus, preempt=0, mirror=False):
+def _env(cus, preempt=0):
     env = dict(os.environ)
     env.update({"LD_PRELOAD": emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0",
                 "EMU_CUS": str(cus), "EMU_PREEMPT": str(preempt)})
-    if mirror:
-        env["GP_EXPERIMENTAL_MIRROR"] = "1"
     return env
 
 
@@ -51,8 +49,8 @@ def _trace(name, cap, n, res, lds=16):
 # time), so the module starts them all at once and each test waits for its own.
 CASES = {
     "c1_interleaved": (_trace("c1", 4096, 4096, 32), _env(2, preempt=3)),
-    "mirror_lds": (_trace("mirror", 4096, 2048, 32, 16), _env(2, preempt=3, mirror=True)),
-    "mirror_spill": (_trace("mirror", 4096, 2048, 32, 8), _env(2, preempt=3, mirror=True)),
+    "mirror_lds": (_trace("mirror", 4096, 2048, 32, 16), _env(2, preempt=3)),
+    "mirror_spill": (_trace("mirror", 4096, 2048, 32, 8), _env(2, preempt=3)),
     "splat_flush": (_trace("c0", 1 << 14, 1 << 14, 128), _env(1)),
     "device_bvh": ([sys.executable, os.path.join(ROOT, "tests", "emu", "run_lbvh.py")], _env(2)),
 }

@@ -10,14 +10,7 @@ import pytest
 import goodphotons_amd as gp
 from goodphotons_amd import scenes
 
-# The mirror path faulted the GPU on its first trace (mirror scene, 64^2, 2^19 photons)
-# and again after a splat-table race was fixed. The cause, found afterwards by reading
-# the kernel: k_mirror ran its any-hit and closest-hit traversals on two views of one
-# LDS array whose per-lane columns overlap across lanes, so concurrent lanes clobbered
-# each other's stacks. The kernel now has separate arrays; until that is verified on the
-# GPU the library rejects Mirror materials (GP_ERR_UNSUPPORTED) and these tests skip.
-MIRROR_OFF = "flat-mirror connection switched off until its LDS-stack fix is verified on the GPU"
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason=MIRROR_OFF)]
+pytestmark = pytest.mark.gpu
 
 MIRROR = 4   # material index of the mirrors in scenes.mirror_box
 

@@ -27,9 +27,7 @@ def _cams(sd):
 @pytest.mark.parametrize("name,make,n,depth,rr", [
     ("c0", lambda: scenes.cornell_c0(48), 1 << 17, 8, 3),
     ("c1", lambda: scenes.caustics_c1(48), 1 << 17, 16, 0),
-    pytest.param("mirror", lambda: scenes.mirror_box(48), 1 << 17, 8, 0,
-                 marks=pytest.mark.skip(reason="flat-mirror connection switched off until its LDS-stack fix "
-                                               "is verified on the GPU")),
+    ("mirror", lambda: scenes.mirror_box(48), 1 << 17, 8, 0),
 ])
 def test_shared_cameras_equal_single_camera_traces(gplib, name, make, n, depth, rr):
     sd = make()
