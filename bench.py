@@ -1,17 +1,25 @@
 """Benchmark: traced segments/s of the forward light tracer on one MI355X.
 
-Workload (the dispersive-caustics configuration): Cornell box + two diffuse boxes
-+ 64x64-segment N-BK7 UV sphere + SF10 prism, 0.04 x 0.04 quad light (6500 K,
-380-780 nm), pinhole 512x512, 2^26 photons per step, max depth 16, hero C = 4,
-float32 transport, int64 fixed-point XYZ film. One step = one full render of the
-2^26 photons (scene and film resident in HBM before the timed region).
+Headline workload (the largest single-GPU configuration): a 5,242,880-triangle
+icosphere (subdivided 9 times, radially displaced by seeded 3-octave value noise,
+amplitude 0.05), Cauchy dielectric n = 1.5 + 0.004/lambda^2, on a diffuse floor under
+a 3200 K quad light; pinhole 1024x1024, 2^28 photons per step, max depth 12, hero
+C = 4, float32 transport, int64 fixed-point XYZ film. One step = one full render of
+the 2^28 photons (scene and film resident in HBM before the timed region). The
+dispersive-caustics workload (512x512, 2^26 photons, depth 16) is timed beside it
+and reported under "c1".
 
 Prints one JSON line. `value` = segments / wall time over the timed steps.
 `roofline` is for the dominant kernel, from HIP events on the engine stream in a
-profiled step of the same workload. `cpu_baseline` times the reference's own CPU
-forward path (oracle/_ref, built from its sources; kind "reference"), or the oracle's
-restatement of it where that build is absent (kind "port"), on a bounded sample of the
-same workload on this host's cores.
+profiled step of the same workload; its algorithmic bytes follow the segment byte
+model below, with the BVH node and leaf reads of the 5.24 M-triangle scene counted
+from the reference tree's traversal statistics (it does not fit the caches).
+`cpu_baseline` times the reference's own CPU forward path (oracle/_ref, built from
+its sources; kind "reference"), or the oracle's restatement where that build is
+absent (kind "port"), on a bounded sample of the same workload on this host's cores.
+
+  python bench.py [--workload c2|c1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py --traversal-stats OUT.json   (per-configuration traversal counts)
 """
 from __future__ import annotations
 
@@ -28,110 +36,179 @@ HBM_PEAK_GBPS = 8000.0
 
 # Algorithmic bytes per work item of each kernel (structure-of-arrays records):
 #   extend : read origin, direction, state (3 x 16 B), write hit (8 B)        per segment
+#            + scene reads where the scene exceeds the caches: 32 B per node visit
+#              and 48 B per leaf test of the reference's binary tree (V_n, V_p)
 #   shade  : read path (80 B) + hit (8 B) + triangle normal (16 B)            per segment
 #            write survivor path (80 B)                                       per survivor
-#            write connection (48 B)                                          per connection
+#            write connection (48 B)                                          per shaded connection
 #   emit   : write path (80 B)                                                per photon
+#            write connection (48 B)                                          per direct connection
 #   connect: read connection (48 B)                                           per connection
+#            + scene reads as extend (V'_n, V'_p of the shadow rays)
 #            film X, Y, Z, hits read-modify-write (4 x 8 B)                   per splat
 EXTEND_B_PER_SEG = 56
 SHADE_B_PER_SEG = 104
 PATH_B = 80
 CONN_B = 48
 SPLAT_B = 32
+NODE_B = 32      # one binary node visit: a box (24 B) and its links (8 B) in float32
+TRI_B = 48       # one leaf test: three float32 vertices + padding
 
 
-def kernel_bytes(st):
+def kernel_bytes(st, trav=None):
+    """Algorithmic bytes of one trace by kernel. `trav` (scene-resident traffic, only
+    for a scene larger than the caches): per-segment V_n, V_p and per-connection
+    V'_n, V'_p of the reference's binary tree."""
     seg, ph, conn, spl = st["segments"], st["photons"], st["connections"], st["splats"]
+    direct = st.get("direct_connections", 0)
     survivors = max(0, seg - ph)
+    ext = seg * EXTEND_B_PER_SEG
+    con = conn * CONN_B + spl * SPLAT_B
+    if trav:
+        ext += seg * (trav["V_n"] * NODE_B + trav["V_p"] * TRI_B)
+        con += conn * (trav["Va_n"] * NODE_B + trav["Va_p"] * TRI_B)
     return {
-        "k_extend": seg * EXTEND_B_PER_SEG,
-        "k_shade": seg * SHADE_B_PER_SEG + survivors * PATH_B + conn * CONN_B,
-        "k_emit": ph * PATH_B,
-        "k_connect": conn * CONN_B + spl * SPLAT_B,
+        "k_extend": ext,
+        "k_shade": seg * SHADE_B_PER_SEG + survivors * PATH_B + (conn - direct) * CONN_B,
+        "k_emit": ph * PATH_B + direct * CONN_B,
+        "k_connect": con,
         "k_schedule": 0,
     }
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_traffic.json")
+def _workloads():
+    from goodphotons_amd import scenes
+    return {
+        "c2": dict(make=lambda: scenes.icosphere_c2(level=9, res=1024), res=(1024, 1024),
+                   scene_bytes=True,
+                   desc="c2 large mesh: 9x-subdivided icosphere (5,242,880 tris) displaced by seeded "
+                        "3-octave value noise (amplitude 0.05), Cauchy dielectric n=1.5+0.004/um^2, "
+                        "diffuse floor, 3200K quad light, 1024x1024 pinhole, 2^28 photons/step, depth 12, "
+                        "hero C=4, int64 XYZ film"),
+        "c1": dict(make=lambda: scenes.caustics_c1(512), res=(512, 512), scene_bytes=False,
+                   desc="c1 dispersive caustics: Cornell box + 2 boxes + 64x64-segment N-BK7 UV sphere "
+                        "+ SF10 prism, 0.04x0.04 quad light 6500K, 512x512 pinhole, 2^26 photons/step, "
+                        "depth 16, hero C=4, int64 XYZ film"),
+    }
 
 
-def measured_traffic(kernel, tuning, photons):
+PROFILES = os.path.join(ROOT, "profiles")
+TRAV_PROFILE = os.path.join(PROFILES, "traversal_stats.json")
+
+
+def committed_traversal(workload):
+    """Per-segment / per-connection traversal counts of `workload` from the committed
+    measurement (bench.py --traversal-stats): the reference's binary tree (oracle) and
+    this library's four-wide tree (GP_TRACE_TRAVERSAL_STATS)."""
+    try:
+        with open(TRAV_PROFILE) as f:
+            return json.load(f)["configs"].get(workload)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def measured_traffic(workload, kernel, tuning, photons):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes of this workload (scaled by the copy calibration), when they
     were taken at the same tuning and photon count; otherwise None."""
+    path = os.path.join(PROFILES, f"{workload}_traffic.json")
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
-        return None
-    if photons != 1 << 26 or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
-        return None
-    return rec["kernels"][kernel]["hbm_bytes_per_dispatch"]
+        return None, None
+    if rec.get("photons") != photons or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
+        return None, None
+    return rec["kernels"][kernel]["hbm_bytes_per_dispatch"], os.path.relpath(path, ROOT)
 
 
-VALU_PROFILE = os.path.join(ROOT, "profiles", "c1_valu.json")
 # wave64 VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
 
 
-def measured_valu(kernel, tuning, photons, avg_ms):
+def measured_valu(workload, kernel, tuning, photons, avg_ms):
     """VALU issue of `kernel` against the wave64 issue peak, from the committed
     rocprofv3 SQ_INSTS_VALU / SQ_THREAD_CYCLES_VALU pass of this workload (same
     tuning and photon count), divided by this run's live average launch time."""
+    path = os.path.join(PROFILES, f"{workload}_valu.json")
     try:
-        with open(VALU_PROFILE) as f:
+        with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if photons != 1 << 26 or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
+    if rec.get("photons", 1 << 26) != photons or rec.get("tuning") != tuning or kernel not in rec.get("kernels", {}):
         return None
     k = rec["kernels"][kernel]
     rate = k["valu_insts_per_dispatch"] / (avg_ms * 1e-3) / 1e9
     return {"bound": "valu_issue", "achieved": rate, "peak": VALU_PEAK_G, "unit": "G wave-instructions/s",
             "frac": rate / VALU_PEAK_G, "lane_utilization": k["lane_utilization"],
-            "source": "profiles/c1_valu.json"}
+            "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_threads():
+    """Threads for the CPU baseline: every core this process may run on, except
+    where the host sets a CPU share per GPU through OMP_NUM_THREADS (a shared GPU
+    host: more threads would take other GPUs' cores). Returns (threads, host facts)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(share) if share.isdigit() and int(share) > 0 else None
+    t = min(aff, share) if share else aff
+    return max(1, t), {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": share}
 
 
-def cpu_baseline(sd, target_s=10.0):
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sd, res, target_s=12.0):
     """Time the CPU forward path on a bounded sample of the same workload.
 
     With the reference build present (oracle/_ref), the reference itself is timed
     (kind "reference"); the oracle traces the same photons to count segments exactly
-    (its ray-query total must equal the reference's, a live bit-identity check).
-    Otherwise the oracle is timed (kind "port")."""
+    (its ray-query total must equal the reference's, a live bit-identity check) and
+    the traversal statistics of the reference's binary tree. Otherwise the oracle is
+    timed (kind "port"). Scene construction and BVH builds are outside the timing."""
     import oracle
     if not oracle.available("oracle"):
         return None
     orc = oracle.Oracle()
+    _log("cpu baseline: oracle scene + BVH")
     cs = orc.scene(sd)
-    T = cpu_threads()
+    T, host = cpu_threads()
     cam = sd.camera
     kw = dict(hero_c=sd.hero_c, max_bounce=sd.max_depth)
     ref = oracle.Reference() if oracle.available("ref") and sd.rr_depth == 0 else None
     timer = ref if ref is not None else orc
+    if ref is not None:
+        _log("cpu baseline: reference scene + BVH")
     ts = timer.scene(sd) if ref is not None else cs
-    pilot = 1 << 16
+    pilot = 1 << 18
     t0 = time.perf_counter()
     timer.render(ts, cam, pilot, seed_base=1 << 41, threads=T, **kw)
     dt = time.perf_counter() - t0
-    n = int(min(1 << 25, max(1 << 16, pilot * target_s / max(dt, 1e-3))))
+    n = int(min(1 << 26, max(1 << 18, pilot * target_s / max(dt, 1e-3))))
+    _log(f"cpu baseline: timing {n} photons on {T} threads")
     t0 = time.perf_counter()
     _, _, _, tst = timer.render(ts, cam, n, seed_base=1 << 40, threads=T, **kw)
     dt = time.perf_counter() - t0
+    _log("cpu baseline: oracle on the same photons")
     _, _, _, st = orc.render(cs, cam, n, seed_base=1 << 40, threads=T, rr_depth=sd.rr_depth, **kw)
-    out = {"value": st["segments"] / dt, "unit": "segments/s", "cores": T,
+    v = st["segments"] / dt
+    out = {"value": v, "unit": "segments/s", "cores": T,
            "kind": "reference" if ref is not None else "port",
-           "sample": f"{n} photons of the same workload (512x512, depth {sd.max_depth}, hero C={sd.hero_c}), "
-                     f"{st['segments']} segments in {dt:.2f} s on {T} threads"}
+           "sample": f"{n} photons of the same workload ({res[0]}x{res[1]}, depth {sd.max_depth}, "
+                     f"hero C={sd.hero_c}), {st['segments']} segments in {dt:.2f} s on {T} threads",
+           "host": host,
+           "per_core": v / T,
+           "linear_bound_all_affinity_cpus": v / T * host["affinity_cpus"],
+           "traversal_reference_tree": {
+               "V_n": st["node_visits"] / max(1, st["segments"]),
+               "V_p": st["leaf_tests"] / max(1, st["segments"]),
+               "Va_n": st["any_node_visits"] / max(1, st["connections"]),
+               "Va_p": st["any_leaf_tests"] / max(1, st["connections"])}}
     if ref is not None:
         out["reference_ray_queries"] = tst["ray_queries"]
         out["port_ray_queries"] = st["segments"] + st["connections"]
@@ -156,16 +233,93 @@ def copy_calibration(nbytes):
     hip.hipFree(b)
 
 
+def time_workload(lib, sd, res, N, steps, warmup):
+    """Resident scene and film; `steps` full renders of N photons after `warmup`.
+    Returns (segments/s, seconds, segments per step, scene, film, step)."""
+    import goodphotons_amd as gp
+    gs = gp.GpuScene(sd, lib)
+    film = gp.Film(res[0], res[1], max_photons=N, lib=lib)
+
+    def step(seed_base, stats=False):
+        film.reset()
+        e, st = gp.trace(gs, sd.camera, film, N, seed_base=seed_base, hero_c=sd.hero_c,
+                         max_depth=sd.max_depth, rr_depth=sd.rr_depth, traversal_stats=stats)
+        return st.as_dict()
+
+    for w in range(warmup):
+        step(w * N)
+    segs = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        segs += step((warmup + k) * N)["segments"]
+    dt = time.perf_counter() - t0
+    return segs / dt, dt, segs / max(1, steps), gs, film, step
+
+
+def traversal_stats(out_path):
+    """Per-configuration traversal counts: the reference's binary tree (the oracle's
+    TraversalStats over the segments and shadow rays of a photon sample) and this
+    library's four-wide tree (GP_TRACE_TRAVERSAL_STATS on the same photons)."""
+    import goodphotons_amd as gp
+    import oracle
+    from goodphotons_amd import scenes
+    lib = gp.load()
+    orc = oracle.Oracle()
+    T, _ = cpu_threads()
+    cfgs = {"c0": (lambda: scenes.cornell_c0(256), 1 << 20),
+            "c1": (lambda: scenes.caustics_c1(512), 1 << 20),
+            "c2": (lambda: scenes.icosphere_c2(level=9, res=1024), 1 << 20),
+            "soup1M": (lambda: scenes.soup(1_000_000, 3, res=512), 1 << 20)}
+    res = {}
+    for name, (make, n) in cfgs.items():
+        _log(f"traversal stats: {name}")
+        sd = make()
+        kw = dict(hero_c=sd.hero_c, max_bounce=sd.max_depth, rr_depth=sd.rr_depth)
+        _, _, _, c = orc.render(orc.scene(sd), sd.camera, n, threads=T, **kw)
+        gs = gp.GpuScene(sd, lib)
+        film = gp.Film(sd.camera.res_x, sd.camera.res_y, max_photons=n, lib=lib)
+        _, g = gp.trace(gs, sd.camera, film, n, hero_c=sd.hero_c, max_depth=sd.max_depth,
+                        rr_depth=sd.rr_depth, traversal_stats=True)
+        g = g.as_dict()
+        film.close()
+        info = gs.info()
+        res[name] = {
+            "photons": n, "tris": sd.n_tris,
+            "reference_tree": {"V_n": c["node_visits"] / c["segments"], "V_p": c["leaf_tests"] / c["segments"],
+                               "Va_n": c["any_node_visits"] / max(1, c["connections"]),
+                               "Va_p": c["any_leaf_tests"] / max(1, c["connections"]),
+                               "segments": c["segments"], "connections": c["connections"]},
+            "four_wide_tree": {"V_n": g["node_visits"] / g["segments"], "V_p": g["tri_tests"] / g["segments"],
+                               "Va_n": g["shadow_node_visits"] / max(1, g["connections"]),
+                               "Va_p": g["shadow_tri_tests"] / max(1, g["connections"]),
+                               "segments": g["segments"], "connections": g["connections"],
+                               "direct_connections": g["direct_connections"],
+                               "nodes": info["n_nodes"], "device_bytes": info["device_bytes"]},
+        }
+        ref, fw = res[name]["reference_tree"], res[name]["four_wide_tree"]
+        # bytes per closest-hit query: binary model (32 B nodes) vs four-wide (128 B nodes)
+        res[name]["node_bytes_factor"] = (fw["V_n"] * 128) / (ref["V_n"] * NODE_B)
+    rec = {"command": "python bench.py --traversal-stats " + os.path.relpath(out_path, ROOT),
+           "node_bytes": {"binary": NODE_B, "four_wide": 128}, "tri_bytes": TRI_B, "configs": res}
+    with open(out_path, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--photons", type=int, default=1 << 26)
+    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--photons", type=int, default=0, help="photons per step (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the c1 line beside the c2 headline")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
+    ap.add_argument("--traversal-stats", default="",
+                    help="only measure per-configuration traversal counts into this JSON file")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -176,47 +330,44 @@ def main():
               "plus one int64 film all-reduce is designed but not built); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.traversal_stats:
+        traversal_stats(args.traversal_stats)
+        return
 
     import goodphotons_amd as gp
-    from goodphotons_amd import scenes
 
     lib = gp.load()
     dev = lib.device_info()
-    sd = scenes.caustics_c1(512)
-    N = args.photons
-    gs = gp.GpuScene(sd, lib)
-    info = gs.info()
+    W = _workloads()
+    wl = W[args.workload]
+    _log(f"{args.workload}: building the scene")
+    sd = wl["make"]()
+    N = args.photons or sd.n_photons
+    res = wl["res"]
     if args.autotune:
         import ctypes as C
+        gs0 = gp.GpuScene(sd, lib)
         t = gp.abi.Tuning()
         p = gp.trace_params(N, hero_c=sd.hero_c, max_depth=sd.max_depth, rr_depth=sd.rr_depth)
-        lib.check(lib.lib.gp_autotune(gs.h, C.byref(sd.camera), C.byref(p), C.byref(t)))
-    film = gp.Film(512, 512, max_photons=N, lib=lib)
-
-    def step(seed_base):
-        film.reset()
-        e, st = gp.trace(gs, sd.camera, film, N, seed_base=seed_base, hero_c=sd.hero_c,
-                         max_depth=sd.max_depth, rr_depth=sd.rr_depth)
-        return st.as_dict()
-
-    for w in range(args.warmup):
-        step(w * N)
-    segs = 0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        st = step((args.warmup + k) * N)
-        segs += st["segments"]
-    dt = time.perf_counter() - t0
-    value = segs / dt
+        lib.check(lib.lib.gp_autotune(gs0.h, C.byref(sd.camera), C.byref(p), C.byref(t)))
+        gs0.close()
+    _log(f"{args.workload}: timing {args.steps} steps of {N} photons")
+    value, dt, seg_step, gs, film, step = time_workload(lib, sd, res, N, args.steps, args.warmup)
+    info = gs.info()
 
     # Profiled step: per-kernel HIP-event times on the engine stream.
     tun = lib.tuning()
+    tund = {k: getattr(tun, k) for k, _ in tun._fields_}
     lib.set_tuning(profile=1)
     try:
         pst = step(0)
     finally:
         lib.set_tuning(profile=0)
-    kb = kernel_bytes(pst)
+    # Four-wide traversal counts of the same photons (counting kernel variants).
+    tst = step(0, stats=True)
+    trav_c = committed_traversal(args.workload)
+    trav = trav_c["reference_tree"] if (trav_c and wl["scene_bytes"]) else None
+    kb = kernel_bytes(pst, trav)
     kms = dict(zip(gp.abi.KERNEL_NAMES, pst["kernel_ms"]))
     kl = dict(zip(gp.abi.KERNEL_NAMES, pst["kernel_launches"]))
     dom = max(kms, key=kms.get)
@@ -225,34 +376,59 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     total_bytes = sum(kb.values())
     total_ms = sum(kms.values())
-    traffic = measured_traffic(dom, {k: getattr(tun, k) for k, _ in tun._fields_}, N)
+    traffic, tsrc = measured_traffic(args.workload, dom, tund, N)
+    seg = max(1, tst["segments"])
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                "traffic_source": "profiles/c1_traffic.json" if traffic is not None else None,
-                "avg_launch_ms": avg_ms, "launches": kl[dom],
+                "traffic_source": tsrc, "avg_launch_ms": avg_ms, "launches": kl[dom],
                 "algorithmic_bytes_per_launch": bytes_per_launch,
-                "valu": measured_valu(dom, {k: getattr(tun, k) for k, _ in tun._fields_}, N, avg_ms),
+                "algorithmic_bytes_per_segment": kb[dom] / max(1, pst["segments"]),
+                "scene_traffic_model": (
+                    {"source": "profiles/traversal_stats.json (oracle TraversalStats, reference binary tree)",
+                     **trav} if trav else "scene cache-resident: queue traffic only"),
+                "four_wide_traversal": {"node_visits_per_segment": tst["node_visits"] / seg,
+                                        "tri_tests_per_segment": tst["tri_tests"] / seg,
+                                        "shadow_node_visits_per_connection":
+                                            tst["shadow_node_visits"] / max(1, tst["connections"]),
+                                        "shadow_tri_tests_per_connection":
+                                            tst["shadow_tri_tests"] / max(1, tst["connections"])},
+                "valu": measured_valu(args.workload, dom, tund, N, avg_ms),
                 "kernel_ms": kms, "pipeline_bytes_per_segment": total_bytes / max(1, pst["segments"]),
                 "pipeline_GBps": total_bytes / (total_ms * 1e-3) / 1e9}
+    film.close()
+    gs.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(sd)
+    extra = None
+    if args.workload == "c2" and not args.no_extra:
+        w1 = W["c1"]
+        sd1 = w1["make"]()
+        _log(f"c1: timing {args.steps} steps of {sd1.n_photons} photons")
+        v1, dt1, seg1, gs1, film1, _ = time_workload(lib, sd1, w1["res"], sd1.n_photons, args.steps, args.warmup)
+        extra = {"workload": w1["desc"], "value": v1, "unit": "segments/s",
+                 "ms_per_step": dt1 * 1e3 / args.steps, "segments_per_step": seg1}
+        film1.close()
+        gs1.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(sd, res)
     out = {
         "metric": "traced segments/s at matched 1% image RMSE",
         "value": value, "unit": "segments/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c1 dispersive caustics: Cornell box + 2 boxes + 64x64-segment N-BK7 "
-                               "UV sphere + SF10 prism, 0.04x0.04 quad light 6500K, 512x512 pinhole, "
-                               "2^26 photons/step, depth 16, hero C=4, int64 XYZ film",
-                   "photons_per_step": N, "tris": info["n_tris"], "bvh_depth": info["bvh_depth"],
-                   "segments_per_step": segs / args.steps,
+        "config": {"workload": wl["desc"], "photons_per_step": N, "tris": info["n_tris"],
+                   "bvh_depth": info["bvh_depth"], "scene_device_bytes": info["device_bytes"],
+                   "segments_per_step": seg_step,
                    "device": f"{dev['name'] or 'AMD GPU'} ({dev['arch']}, {dev['compute_units']} CUs)",
-                   "tuning": {k: getattr(tun, k) for k, _ in tun._fields_},
+                   "tuning": tund,
                    "matched_rmse": "GPU and CPU trace the same photons (same per-photon PCG "
-                                   "streams), so equal photon count is matched RMSE"},
+                                   "streams), so equal photon count is matched RMSE; the GPU films "
+                                   "are checked against the oracle's at the c0 and c1 bench sizes "
+                                   "(relative RMSE < 1%, mean luminance within 0.3%)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if extra:
+        out["c1"] = extra
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)

@@ -297,7 +297,8 @@ def write_png(path, rgb8):
         f.write(chunk(b"IEND", b""))
 
 
-def trace_params(n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4, max_depth=32, rr_depth=0):
+def trace_params(n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4, max_depth=32, rr_depth=0,
+                 traversal_stats=False):
     p = abi.TraceParams()
     p.n_photons = int(n)
     p.seed_base = int(seed_base)
@@ -305,13 +306,15 @@ def trace_params(n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4, max_depth=32
     p.hero_c = int(hero_c)
     p.max_depth = int(max_depth)
     p.rr_depth = int(rr_depth)
-    p.flags = 0
+    p.flags = abi.TRACE_TRAVERSAL_STATS if traversal_stats else 0
     return p
 
 
 def trace(scene: GpuScene, cam, film: Film, n, seed_base=0, salt=abi.REFERENCE_SALT, hero_c=4,
-          max_depth=32, rr_depth=0, energy: abi.Energy | None = None):
-    p = trace_params(n, seed_base, salt, hero_c, max_depth, rr_depth)
+          max_depth=32, rr_depth=0, energy: abi.Energy | None = None, traversal_stats=False):
+    """gp_trace; traversal_stats=True also counts node visits and triangle tests
+    (GP_TRACE_TRAVERSAL_STATS) into the returned stats."""
+    p = trace_params(n, seed_base, salt, hero_c, max_depth, rr_depth, traversal_stats)
     e = energy if energy is not None else abi.Energy()
     st = abi.TraceStats()
     scene.L.check(scene.L.lib.gp_trace(scene.h, C.byref(cam), film.h, C.byref(p), C.byref(e), C.byref(st)))

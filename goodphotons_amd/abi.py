@@ -62,7 +62,10 @@ class TraceParams(C.Structure):
 class TraceStats(C.Structure):
     _fields_ = [("photons", C.c_uint64), ("segments", C.c_uint64), ("connections", C.c_uint64),
                 ("splats", C.c_uint64), ("iterations", C.c_uint64), ("wall_ms", C.c_double),
-                ("kernel_ms", C.c_double * 5), ("kernel_launches", C.c_uint64 * 5)]
+                ("kernel_ms", C.c_double * 5), ("kernel_launches", C.c_uint64 * 5),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("direct_connections", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -111,6 +114,7 @@ class JobResult(C.Structure):
 
 
 REFERENCE_SALT = 0x9E3779B97F4A7C15
+TRACE_TRAVERSAL_STATS = 1           # gp_trace_params.flags: GP_TRACE_TRAVERSAL_STATS
 
 # Every symbol include/goodphotons.h declares; the CPU tests check the built
 # library exports each one.

@@ -436,12 +436,19 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (p->hero_c < 1 || p->hero_c > kMaxHero) return fail(GP_ERR_INVALID, "hero_c must be 1..4");
     if (p->max_depth < 1 || p->max_depth > 250) return fail(GP_ERR_INVALID, "max_depth must be 1..250");
     if (p->rr_depth < 0) return fail(GP_ERR_INVALID, "negative rr_depth");
+    if (p->flags & ~GP_TRACE_TRAVERSAL_STATS) return fail(GP_ERR_INVALID, "unknown trace flags");
     gp_status st = requireDevice();
     if (st != GP_OK) return st;
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
     if (E.owner)
         return fail(GP_ERR_STATE, "a server has jobs in flight on this device; step it until they finish first");
+    // the statistics variants run for this call only (plain launches, no graph)
+    struct StatsScope {
+        Engine& E;
+        ~StatsScope() { E.q.stats = 0; }
+    } statsScope{E};
+    E.q.stats = (p->flags & GP_TRACE_TRAVERSAL_STATS) ? 1 : 0;
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     ++E.epoch;
     const int lds = E.tun.lds_stack;
@@ -492,17 +499,14 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
             if (C.nextPhoton[0] >= (unsigned long long)n && live == 0) break;
             if (iters > maxIters) return fail(GP_ERR_STATE, "wavefront did not drain");
         }
-        unsigned long long ev[5], cv[4];
+        unsigned long long ev[5], cv[kAccCounters];
         if ((st = E.readJobAcc(0, ev, cv)) != GP_OK) return st;
         eacc.emitted += (double)ev[0] / J.energyScale;
         eacc.absorbed += (double)ev[1] / J.energyScale;
         eacc.sensor += (double)ev[2] / J.energyScale;
         eacc.escaped += (double)ev[3] / J.energyScale;
         eacc.residual += (double)ev[4] / J.energyScale;
-        acc.photons += cv[0];
-        acc.segments += cv[1];
-        acc.connections += cv[2];
-        acc.splats += cv[3];
+        addCounters(cv, acc);
         acc.iterations += C.iter;
         done += n;
     }

@@ -367,6 +367,23 @@ __device__ __forceinline__ int visit4Any(const RayPre& r, const float4* __restri
     return (h[0] ? 1 : 0) + (h[1] ? 1 : 0) + (h[2] ? 1 : 0) + (h[3] ? 1 : 0);
 }
 
+// Traversal statistics of one query (the reference's TraversalStats, src/bvh.h:79,
+// for the four-wide tree): node visits = four-wide nodes loaded and slab-tested (a
+// leaf is a link in its parent, not a node), triangle tests = triangles intersected.
+// Counted only by the statistics variants of the kernels; TravCount<false> compiles
+// away.
+template <bool ON>
+struct TravCount {
+    __device__ void node() {}
+    __device__ void tris(int) {}
+};
+template <>
+struct TravCount<true> {
+    unsigned nodes = 0, tri = 0;
+    __device__ void node() { ++nodes; }
+    __device__ void tris(int c) { tri += (unsigned)c; }
+};
+
 template <int SD, bool SPILL>
 __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax, int& node) {
     while (stk.sp > 0) {
@@ -381,10 +398,10 @@ __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax,
 // and its distance. Front-to-back: the nearest hit slot is visited next and the
 // others pushed farthest first with their entry distances; a popped entry farther
 // than the current hit is skipped. Loop shape: while-while (Aila & Laine 2009).
-template <int SD, bool SPILL>
+template <int SD, bool SPILL, class Cnt>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
-                                            float tmin, float& tmax, Stack<SD, true, SPILL>& stk) {
+                                            float tmin, float& tmax, Stack<SD, true, SPILL>& stk, Cnt& cnt) {
     int best = -1;
     int node = 0;
     stk.sp = 0;
@@ -394,6 +411,7 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
             float t[4];
             int l[4];
             const int n = visit4(r, nodes + 8 * node, tmin, tmax, t, l);
+            cnt.node();
             if (!SPILL) {
                 stk.e2[stk.sp * kTravThreads] = make_int2(l[3], __float_as_int(t[3]));
                 stk.sp += n > 3 ? 1 : 0;
@@ -418,6 +436,7 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
         const int first = code >> 3, count = (code & 7) + 1;
         LeafTris L;
         loadLeaf(tris, first, count, L);
+        cnt.tris(count);
 #pragma unroll
         for (int k = 0; k < kLeafMax; ++k) {
             float t;
@@ -433,10 +452,10 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
 
 // Any hit on [tmin, tmax) over the four-wide tree. Same loop shape; hit slots are
 // visited lowest slot first.
-template <int SD, bool SPILL>
+template <int SD, bool SPILL, class Cnt>
 __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
                                          const float4* __restrict__ tris, const RayPre& r,
-                                         float tmin, float tmax, Stack<SD, false, SPILL>& stk) {
+                                         float tmin, float tmax, Stack<SD, false, SPILL>& stk, Cnt& cnt) {
     int node = 0;
     stk.sp = 0;
     while (true) {
@@ -445,6 +464,7 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
             bool h[4];
             int l[4];
             const int n = visit4Any(r, nodes + 8 * node, tmin, tmax, h, l);
+            cnt.node();
             // The lowest hit slot is visited next; each higher hit is pushed when a lower
             // one turns up. Without spill the write is unconditional: the stack top
             // advances at most twice before the last write, so it stays within S.
@@ -480,7 +500,10 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
 #pragma unroll
         for (int k = 0; k < kLeafMax; ++k) {
             float t;
-            if (k < count && !hit) hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
+            if (k < count && !hit) {
+                cnt.tris(1);
+                hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
+            }
         }
         if (hit) return true;
         if (stk.sp == 0) break;

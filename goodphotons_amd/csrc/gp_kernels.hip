@@ -93,8 +93,6 @@ struct AccAddr {
         return jobs[j].acc + (size_t)(blockIdx.x % kSegs) * kAccRow + base + k;
     }
 };
-constexpr int kAccEnergy = 0;     // emitted, absorbed, sensor, escaped, residual
-constexpr int kAccPhotons = 5;    // then segments, connections, splats
 
 // Logical index -> slot of a segmented queue. `pre` is the prefix of the segment
 // counts (wave-uniform); the segment is the last one whose prefix is <= i, which
@@ -179,10 +177,12 @@ __device__ __forceinline__ bool makeConnection(const DevCamera& K, unsigned tagJ
 
 // Connections of one vertex to every camera of its job, appended to the workgroup's
 // segment of the connection queue. Called by all lanes of the wave (nCams = 0 on
-// lanes without a vertex); the loop runs while any lane has a camera left.
-__device__ __forceinline__ void connectCameras(const Queues& q, Ctrl* c, const DevCamera* cams, int nCams,
-                                               unsigned job, F3 p, F3 n, const float* lam,
-                                               const float* beta, const float* rho, int nUp) {
+// lanes without a vertex); the loop runs while any lane has a camera left. Returns
+// the lane's number of connections.
+__device__ __forceinline__ unsigned connectCameras(const Queues& q, Ctrl* c, const DevCamera* cams, int nCams,
+                                                   unsigned job, F3 p, F3 n, const float* lam,
+                                                   const float* beta, const float* rho, int nUp) {
+    unsigned made = 0;
     for (int k = 0; __any(k < nCams); ++k) {
         float4 co, cd, cx;
         bool conn = false;
@@ -192,8 +192,10 @@ __device__ __forceinline__ void connectCameras(const Queues& q, Ctrl* c, const D
             q.co[ci] = co;
             q.cd[ci] = cd;
             q.cx[ci] = cx;
+            ++made;
         }
     }
+    return made;
 }
 
 // A vertex to connect to its job's cameras through the scene's flat mirrors, queued
@@ -301,11 +303,11 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         emBase[g] = c->segBase[g];
     }
     em.cap = q.segCap;
-    __shared__ unsigned long long eTab[kMaxJobs], pTab[kMaxJobs];
+    __shared__ unsigned long long eTab[kMaxJobs], pTab[kMaxJobs * 2];
     initJobTable<1>(eTab, nJobs);
-    initJobTable<1>(pTab, nJobs);
+    initJobTable<2>(pTab, nJobs);
     JobAcc<1> eacc;   // emitted
-    JobAcc<1> pacc;   // photons
+    JobAcc<2> pacc;   // photons, direct camera connections
     AccAddr ea{q.jobs, kAccEnergy};
     AccAddr pa{q.jobs, kAccPhotons};
     const unsigned stride = gridDim.x * blockDim.x;
@@ -362,7 +364,7 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         if (!dead) eacc.v[0] += toFixed((double)b * (double)C, J.energyScale);
         {
             const float one[kMaxHero] = {1.f, 1.f, 1.f, 1.f};
-            connectCameras(q, c, J.cams, dead ? 0 : J.nCams, (unsigned)j, org, en, lam, beta, one, C);
+            pacc.v[1] += connectCameras(q, c, J.cams, dead ? 0 : J.nCams, (unsigned)j, org, en, lam, beta, one, C);
             queueMirrorVertex(q, c, !dead && J.nMirrors > 0, (unsigned)j, org, en, lam, beta, one, C);
         }
         F3 o = org + dir * kRayEps;
@@ -390,8 +392,10 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ extend
-template <int SD, bool SPILL>
+// STATS: also count the traversal statistics (node visits, triangle tests) per job.
+template <int SD, bool SPILL, bool STATS>
 __global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
+    constexpr int NA = STATS ? 3 : 1;   // segments[, node visits, triangle tests]
     __shared__ int2 sStk[SD * kTravThreads];
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
@@ -400,10 +404,10 @@ __global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
     Stack<SD, true, SPILL> stk{sStk + threadIdx.x, nullptr,
                                SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
-    __shared__ unsigned long long sTab[kMaxJobs];
-    initJobTable<1>(sTab, nJobs);
-    JobAcc<1> sacc;
-    AccAddr sa{q.jobs, kAccPhotons + 1};
+    __shared__ unsigned long long sTab[kMaxJobs * NA];
+    initJobTable<NA>(sTab, nJobs);
+    JobAcc<NA> sacc;
+    AccAddr sa{q.jobs, kAccSegments};
     for (unsigned li = gtid; li < n; li += stride) {
         const unsigned i = sm.slot(li);
         const uint4 s4 = q.st[cur][i];
@@ -416,10 +420,15 @@ __global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
             const float4 d = q.rd[cur][i];
             RayPre r = prepareRay(xyz(o), xyz(d));
             float tmax = INFINITY;
-            best = traceClosest<SD, SPILL>(J.nodes, J.tris, r, kRayEps, tmax, stk);
+            TravCount<STATS> tc;
+            best = traceClosest<SD, SPILL>(J.nodes, J.tris, r, kRayEps, tmax, stk, tc);
             t = tmax;
             accumulate(sacc, j, sTab);
             sacc.v[0] += 1;
+            if constexpr (STATS) {
+                sacc.v[1] += tc.nodes;
+                sacc.v[2] += tc.tri;
+            }
         }
         q.hit[i] = make_float2(t, __int_as_float(best));
     }
@@ -673,8 +682,9 @@ __device__ __forceinline__ void clearSplats(unsigned long long* sKey, unsigned l
     }
 }
 
-template <int SD, bool SPILL>
+template <int SD, bool SPILL, bool STATS>
 __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
+    constexpr int NA = STATS ? 4 : 2;   // connections, splats[, node visits, triangle tests]
     __shared__ int sStk[SD * kTravThreads];
     __shared__ unsigned long long sKey[kSplatSlots];
     __shared__ unsigned long long sVal[kSplatSlots * 4];
@@ -686,10 +696,10 @@ __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
     Stack<SD, false, SPILL> stk{nullptr, sStk + threadIdx.x,
                                 SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
-    __shared__ unsigned long long cTab[kMaxJobs * 2];
-    initJobTable<2>(cTab, nJobs);   // (its barrier also orders clearSplats)
-    JobAcc<2> cacc;   // connections, splats
-    AccAddr ca{q.jobs, kAccPhotons + 2};
+    __shared__ unsigned long long cTab[kMaxJobs * NA];
+    initJobTable<NA>(cTab, nJobs);   // (its barrier also orders clearSplats)
+    JobAcc<NA> cacc;
+    AccAddr ca{q.jobs, kAccConnections};
     // Rounds are workgroup-uniform so the table can be flushed between them.
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const unsigned li = base + threadIdx.x;
@@ -705,7 +715,12 @@ __global__ void __launch_bounds__(kTravThreads) k_connect(Queues q) {
             accumulate(cacc, j, cTab);
             cacc.v[0] += 1;
             RayPre r = prepareRay(xyz(co), xyz(cd));
-            const bool occ = traceAny<SD, SPILL>(J.nodes, J.tris, r, kRayEps, co.w, stk);
+            TravCount<STATS> tc;
+            const bool occ = traceAny<SD, SPILL>(J.nodes, J.tris, r, kRayEps, co.w, stk, tc);
+            if constexpr (STATS) {
+                cacc.v[2] += tc.nodes;
+                cacc.v[3] += tc.tri;
+            }
             if (!occ) {
                 cacc.v[1] += 1;
                 const DevCamera& K = J.cams[cam];
@@ -836,8 +851,9 @@ __global__ void __launch_bounds__(kTravThreads) k_mirror(Queues q) {
     const unsigned n = loadSegMap(q.ctrl->mirrorCount, q.mirrorSegCap, sm);
     const int nJobs = q.ctrl->nJobs;
     initJobTable<2>(cTab, nJobs);
-    JobAcc<2> cacc;   // ray queries of the connections, splats
-    AccAddr ca{q.jobs, kAccPhotons + 2};
+    JobAcc<2> cacc;   // ray queries of the connections, splats (legs not in the statistics)
+    AccAddr ca{q.jobs, kAccConnections};
+    TravCount<false> tc;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     int2* spill = SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr;
@@ -865,14 +881,14 @@ __global__ void __launch_bounds__(kTravThreads) k_mirror(Queues q) {
                             (float)(p[2] + g.wP[2] * 1e-6)};
                 Stack<SD, false, SPILL> as{nullptr, sStkAny + threadIdx.x, spill, 0};
                 if (traceAny<SD, SPILL>(J.nodes, J.tris, prepareRay(ol, wl), kRayEps,
-                                        (float)((g.dP - 2e-6) - 1e-6), as))
+                                        (float)((g.dP - 2e-6) - 1e-6), as, tc))
                     continue;
                 cacc.v[0] += 1;
                 Stack<SD, true, SPILL> cs{sStk + threadIdx.x, nullptr, spill, 0};
                 float th = INFINITY;
                 const int tri = traceClosest<SD, SPILL>(
                     J.nodes, J.tris, prepareRay(load3(K.eye), F3{(float)g.wE[0], (float)g.wE[1], (float)g.wE[2]}),
-                    kRayEps, th, cs);
+                    kRayEps, th, cs, tc);
                 if (tri < 0 || fabs((double)th - g.dE) > 1e-4 * (1.0 + g.dE)) continue;
                 const DevMaterial& M = J.mats[__float_as_int(J.trin[tri].w)];
                 if (M.type != 2) continue;   // only Mirror is a planar mirror here
@@ -921,7 +937,8 @@ __global__ void __launch_bounds__(kTravThreads) k_probe_closest(const float4* no
         const float* rp = rays + 6 * i;
         RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
         float tmax = INFINITY;
-        int best = traceClosest<SD, SPILL>(nodes, tris, r, tmin, tmax, stk);
+        TravCount<false> tc;
+        int best = traceClosest<SD, SPILL>(nodes, tris, r, tmin, tmax, stk, tc);
         prim[i] = best >= 0 ? __float_as_int(tris[3 * best].w) : -1;
         tout[i] = best >= 0 ? tmax : INFINITY;
     }
@@ -940,7 +957,8 @@ __global__ void __launch_bounds__(kTravThreads) k_probe_any(const float4* nodes,
     for (long long i = gtid; i < n; i += stride) {
         const float* rp = rays + 6 * i;
         RayPre r = prepareRay(F3{rp[0], rp[1], rp[2]}, F3{rp[3], rp[4], rp[5]});
-        occ[i] = traceAny<SD, SPILL>(nodes, tris, r, tmin, tmax[i], stk) ? 1 : 0;
+        TravCount<false> tc;
+        occ[i] = traceAny<SD, SPILL>(nodes, tris, r, tmin, tmax[i], stk, tc) ? 1 : 0;
     }
 }
 
@@ -1008,13 +1026,19 @@ void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s)
     } while (0)
 
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
-    GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
+#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
+    if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLS);
+    else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+#undef GP_CALLS
 #undef GP_CALL
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
-#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
-    GP_BY_STACK(q.connLds, q.connUseSpill, GP_CALL);
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP, false>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
+#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP, true>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)
+    if (q.stats) GP_BY_STACK(q.connLds, q.connUseSpill, GP_CALLS);
+    else GP_BY_STACK(q.connLds, q.connUseSpill, GP_CALL);
+#undef GP_CALLS
 #undef GP_CALL
 }
 void launchMirror(const Queues& q, const LaunchShape& ls, hipStream_t s) {

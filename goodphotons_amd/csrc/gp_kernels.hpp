@@ -33,6 +33,7 @@ struct Queues {
     int connLds;     // k_connect's LDS stack entries (4-byte entries: at least 24 fit at the
                      // occupancy its registers allow)
     int connUseSpill;
+    int stats;       // 1: k_extend / k_connect count traversal statistics per job
     unsigned segCap, connSegCap, mirrorSegCap;
 };
 

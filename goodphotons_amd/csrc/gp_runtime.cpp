@@ -397,17 +397,33 @@ gp_status Engine::resetJobAcc(int slot) {
     return GP_OK;
 }
 
-gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]) {
+void sumAccRows(const unsigned long long* rows, unsigned long long ledger[5], unsigned long long* counters) {
+    for (int k = 0; k < 5; ++k) ledger[k] = 0;
+    for (int k = 0; k < kAccCounters; ++k) counters[k] = 0;
+    for (int g = 0; g < kSegs; ++g) {
+        for (int k = 0; k < 5; ++k) ledger[k] += rows[g * kAccRow + kAccEnergy + k];
+        for (int k = 0; k < kAccCounters; ++k) counters[k] += rows[g * kAccRow + kAccPhotons + k];
+    }
+}
+
+void addCounters(const unsigned long long* cv, gp_trace_stats& st) {
+    st.photons += cv[kAccPhotons - kAccPhotons];
+    st.direct_connections += cv[kAccDirect - kAccPhotons];
+    st.segments += cv[kAccSegments - kAccPhotons];
+    st.node_visits += cv[kAccNodeVisits - kAccPhotons];
+    st.tri_tests += cv[kAccTriTests - kAccPhotons];
+    st.connections += cv[kAccConnections - kAccPhotons];
+    st.splats += cv[kAccSplats - kAccPhotons];
+    st.shadow_node_visits += cv[kAccAnyNodeVisits - kAccPhotons];
+    st.shadow_tri_tests += cv[kAccAnyTriTests - kAccPhotons];
+}
+
+gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned long long* counters) {
     unsigned long long rows[kSegs * kAccRow];
     GP_HIP(hipMemcpyAsync(rows, dJobAcc + (size_t)slot * kSegs * kAccRow, sizeof rows,
                           hipMemcpyDeviceToHost, device().stream));
     GP_HIP(hipStreamSynchronize(device().stream));
-    for (int k = 0; k < 5; ++k) ledger[k] = 0;
-    for (int k = 0; k < 4; ++k) counters[k] = 0;
-    for (int g = 0; g < kSegs; ++g) {
-        for (int k = 0; k < 5; ++k) ledger[k] += rows[g * kAccRow + k];
-        for (int k = 0; k < 4; ++k) counters[k] += rows[g * kAccRow + 5 + k];
-    }
+    sumAccRows(rows, ledger, counters);
     return GP_OK;
 }
 
@@ -576,7 +592,7 @@ gp_status Engine::collectProfile() {
 gp_status Engine::runIterations(int n) {
     hipStream_t s = device().stream;
     const int K = chunkIterations(tun);
-    if (!tun.use_graph || tun.profile || parity != 0 || n < K) return enqueueIterations(n);
+    if (!tun.use_graph || tun.profile || q.stats || parity != 0 || n < K) return enqueueIterations(n);
     if (!graph || graphIters != K || graphLds != q.lds || graphMirror != mirrorActive) {
         if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
         hipGraph_t g;

@@ -117,7 +117,8 @@ struct Engine {
     DevCamera* hJobCams = nullptr;               // pinned staging for gp_trace
     int camCap = 1;                              // cameras per job the connection queue is sized for
     gp_status resetJobAcc(int slot);             // zero a slot's rows (stream-ordered)
-    gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long counters[4]);
+    // ledger[5] and counters[kAccCounters] (the quantities from kAccPhotons on)
+    gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long* counters);
     void initCtrl();                            // zero the host control block, set capacities
     void* bufs[24] = {};
     int nbufs = 0;
@@ -162,6 +163,11 @@ inline int chunkIterations(const gp_tuning& t) {
     return t.use_graph ? std::max(2, t.iters_per_graph & ~1) : std::max(1, t.iters_per_graph);
 }
 unsigned segmentBound(unsigned cap, unsigned blocks);
+// Sums the kSegs accumulator rows of one job: ledger[5] (fixed point) and
+// counters[kAccCounters] (the quantities from kAccPhotons on).
+void sumAccRows(const unsigned long long* rows, unsigned long long ledger[5], unsigned long long* counters);
+// Adds summed counters into a stats record.
+void addCounters(const unsigned long long* counters, gp_trace_stats& st);
 
 // Job record for slot `slot` over cameras cams[0..nCams) and their films; the camera
 // records (to be placed at J.cams) are written to camsOut.

@@ -239,20 +239,14 @@ gp_status startRetire(Engine& E, gp_server* srv, gp_server::Job& J, hipEvent_t c
 }
 
 void finishRetire(gp_server* srv, gp_server::Job& J) {
-    unsigned long long ev[5] = {}, cv[4] = {};
-    for (int g = 0; g < kSegs; ++g) {
-        for (int k = 0; k < 5; ++k) ev[k] += J.accHost[g * kAccRow + k];
-        for (int k = 0; k < 4; ++k) cv[k] += J.accHost[g * kAccRow + 5 + k];
-    }
+    unsigned long long ev[5] = {}, cv[kAccCounters] = {};
+    sumAccRows(J.accHost, ev, cv);
     J.energy.emitted = (double)ev[0] / J.energyScale;
     J.energy.absorbed = (double)ev[1] / J.energyScale;
     J.energy.sensor = (double)ev[2] / J.energyScale;
     J.energy.escaped = (double)ev[3] / J.energyScale;
     J.energy.residual = (double)ev[4] / J.energyScale;
-    J.stats.photons = cv[0];
-    J.stats.segments = cv[1];
-    J.stats.connections = cv[2];
-    J.stats.splats = cv[3];
+    addCounters(cv, J.stats);
     srv->pool.put(J.accHost);
     J.accHost = nullptr;
     srv->recycle(J.accEv);

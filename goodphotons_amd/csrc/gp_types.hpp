@@ -31,6 +31,18 @@ constexpr int kLeafMax = 4;           // triangles per BVH leaf
 constexpr int kSegs = 8;
 constexpr int kLine = 32;             // u32 per 128-byte line
 constexpr int kAccRow = 16;           // u64 per per-segment ledger/counter row
+// Quantities of a per-job accumulator row (u64 index in the row).
+constexpr int kAccEnergy = 0;         // emitted, absorbed, sensor, escaped, residual (fixed point)
+constexpr int kAccPhotons = 5;        // photons emitted
+constexpr int kAccDirect = 6;         // camera connections queued at the emission points
+constexpr int kAccSegments = 7;       // closest-hit queries of the paths
+constexpr int kAccNodeVisits = 8;     // their four-wide node visits (traversal statistics only)
+constexpr int kAccTriTests = 9;       // their triangle tests (traversal statistics only)
+constexpr int kAccConnections = 10;   // connection ray queries
+constexpr int kAccSplats = 11;        // connections that reached the film
+constexpr int kAccAnyNodeVisits = 12; // camera shadow rays' node visits (traversal statistics only)
+constexpr int kAccAnyTriTests = 13;   // camera shadow rays' triangle tests (traversal statistics only)
+constexpr int kAccCounters = 9;       // counters [kAccPhotons, kAccPhotons + kAccCounters)
 
 // Leaf links are negative: ~((first << 3) | (count - 1)).
 inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (count - 1)); }
@@ -100,9 +112,8 @@ struct DevJob {
     int32_t nCams;
     int32_t nMirrors;        // flat mirror planes of the scene (0: no mirror connections)
     const DevMirrorPlane* mirrors;
-    // kSegs rows of kAccRow u64: [0..4] energy (emitted, absorbed, sensor, escaped,
-    // residual, fixed point), [5..8] counters (photons, segments, connections,
-    // splats); workgroup b adds into row b % kSegs
+    // kSegs rows of kAccRow u64 (layout: the kAcc* indices above); workgroup b adds
+    // into row b % kSegs
     unsigned long long* acc;
     double energyScale;
     // path settings

@@ -163,8 +163,14 @@ typedef struct gp_trace_params {
     int32_t hero_c;          /* wavelengths per path, 1..4                           */
     int32_t max_depth;       /* closest-hit segments per path (Renderer::maxBounce)  */
     int32_t rr_depth;        /* first bounce with Russian roulette; 0 = every vertex */
-    int32_t flags;           /* reserved, 0                                          */
+    int32_t flags;           /* 0, or GP_TRACE_TRAVERSAL_STATS                        */
 } gp_trace_params;
+
+/* gp_trace_params.flags: count the traversal statistics of the trace (the reference's
+ * TraversalStats / -bvhstats, src/bvh.h:79, src/main.cpp:2599) into
+ * gp_trace_stats.node_visits ... shadow_tri_tests. Kernel variants with counters run
+ * instead of the default ones, as plain launches: slower, for measurement. */
+#define GP_TRACE_TRAVERSAL_STATS 1
 
 typedef struct gp_trace_stats {
     uint64_t photons;        /* photons emitted                                      */
@@ -179,6 +185,17 @@ typedef struct gp_trace_stats {
      * index 0 schedule, 1 emit, 2 extend (closest hit), 3 shade, 4 connect. */
     double kernel_ms[5];
     uint64_t kernel_launches[5];
+    /* With GP_TRACE_TRAVERSAL_STATS (else 0), over the four-wide tree: nodes loaded and
+     * slab-tested (a leaf is a link of its parent, not a node) and triangles
+     * intersected by the segments' closest hits, and the same for the camera shadow
+     * rays' any hits (up to the first occluder). Flat-mirror connection legs are not
+     * counted. */
+    uint64_t node_visits;
+    uint64_t tri_tests;
+    uint64_t shadow_node_visits;
+    uint64_t shadow_tri_tests;
+    uint64_t direct_connections; /* of `connections`: those of the emission points (the
+                                    light seen directly), queued by the emission kernel */
 } gp_trace_stats;
 
 typedef struct gp_device_info {

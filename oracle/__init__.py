@@ -289,11 +289,12 @@ class Oracle(_CpuRenderer):
         xyz = np.zeros((npx, 3))
         hits = np.zeros(npx)
         e = abi.Energy()
-        st = np.zeros(3, np.uint64)
+        st = np.zeros(7, np.uint64)
         self.lib.or_render(scene.h, C.byref(cam), n, seed_base, salt, hero_c, max_bounce, rr_depth,
                            threads, _ptr(xyz, _dp), _ptr(hits, _dp), C.byref(e), _ptr(st, _u64p))
-        return xyz, hits, e.as_dict(), {"segments": int(st[0]), "connections": int(st[1]),
-                                       "splats": int(st[2])}
+        keys = ("segments", "connections", "splats", "node_visits", "leaf_tests", "any_node_visits",
+                "any_leaf_tests")
+        return xyz, hits, e.as_dict(), {k: int(v) for k, v in zip(keys, st)}
 
     def closest_hit_margin(self, scene: CpuScene, rays, tmin=1e-6, rel_margin=1e-5):
         rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)

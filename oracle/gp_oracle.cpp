@@ -468,20 +468,27 @@ struct Bvh {
             }
         }
     }
+    // Optional counters (an extension: the reference's traverseAny has none): one
+    // node visit per popped node, plus one for a root that misses, as closest()
+    // counts them; one leaf test per primitive tested.
     template <class Leaf>
-    bool any(const V3& o, const V3& d, double tmin, double tMax, Leaf&& leaf) const {
+    bool any(const V3& o, const V3& d, double tmin, double tMax, Leaf&& leaf, int64_t* visits = nullptr,
+             int64_t* tests = nullptr) const {
         if (nodes.empty()) return false;
         V3 inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
         double tRoot;
-        if (!nodes[0].box.slab(o, inv, tmin, tMax, tRoot)) return false;
+        if (!nodes[0].box.slab(o, inv, tmin, tMax, tRoot)) { if (visits) ++*visits; return false; }
         int stack[64];
         int sp = 0;
         stack[sp++] = 0;
         while (sp) {
+            if (visits) ++*visits;
             const Node& nd = nodes[stack[--sp]];
             if (nd.count > 0) {
-                for (int i = 0; i < nd.count; ++i)
+                for (int i = 0; i < nd.count; ++i) {
+                    if (tests) ++*tests;
                     if (leaf(order[nd.first + i])) return true;
+                }
             } else {
                 double tc;
                 if (nodes[nd.left].box.slab(o, inv, tmin, tMax, tc)) stack[sp++] = nd.left;
@@ -627,14 +634,15 @@ struct Scene {
         }, visits, tests);
         return h;
     }
-    bool occluded(const V3& o, const V3& dir, double maxDist, double tmin = 1e-6) const {
+    bool occluded(const V3& o, const V3& dir, double maxDist, double tmin = 1e-6, int64_t* visits = nullptr,
+                  int64_t* tests = nullptr) const {
         const double seg = maxDist - tmin;
         const Shear sh = makeShear(dir);
         return bvh.any(o, dir, tmin, seg, [&](int prim) {
             Hit h;
             h.t = seg;
             return hitTriangle(sh, o, dir, tris[prim], prim, tmin, h);
-        });
+        }, visits, tests);
     }
 };
 
@@ -677,6 +685,9 @@ struct Energy {
 };
 struct Counters {
     uint64_t segments = 0, connections = 0, splats = 0;
+    // traversal statistics (TraversalStats, bvh.h:79) of the segments' closest-hit
+    // queries and of the camera connections' any-hit queries
+    int64_t nodeVisits = 0, leafTests = 0, anyNodeVisits = 0, anyLeafTests = 0;
 };
 struct Film {
     int resX = 0, resY = 0;
@@ -741,7 +752,7 @@ struct Tracer {
         double cosCam, dist2;
         if (!cam.project(p, px, py, cosCam, dist2)) return;
         ++k.connections;
-        if (S.occluded(p + ng * 1e-6, wdir, dist - 2e-6)) return;
+        if (S.occluded(p + ng * 1e-6, wdir, dist - 2e-6, 1e-6, &k.anyNodeVisits, &k.anyLeafTests)) return;
         ++k.splats;
         double omega = cam.pixelOmega(cosCam);
         double corr = adjointCorr(wi, wdir, n, ng);
@@ -878,7 +889,7 @@ struct Tracer {
         for (int bounce = 0; bounce < P.maxBounce; ++bounce) {
             int nUp = secAlive ? C : 1;
             ++k.segments;
-            Hit h = S.closestHit(ro, rd);
+            Hit h = S.closestHit(ro, rd, 1e-6, &k.nodeVisits, &k.leafTests);
             if (!h.valid) { e.escaped += activeSum(); return; }
             const Mat& m = S.mats[h.mat];
             const bool roulette = bounce >= P.rrDepth;
@@ -1003,7 +1014,7 @@ int or_scene_mirror_planes(void* s, double* out10, int cap) {
 
 int or_render(void* sp, const gp_camera_desc* cd, int64_t N, uint64_t seedBase, uint64_t salt,
               int heroC, int maxBounce, int rrDepth, int nThreads, double* xyz, double* hits,
-              gp_energy* eOut, uint64_t* stats3) {
+              gp_energy* eOut, uint64_t* stats7) {
     const Scene& S = *static_cast<Scene*>(sp);
     Cam cam;
     cam.lookAt(*cd);
@@ -1044,9 +1055,13 @@ int or_render(void* sp, const gp_camera_desc* cd, int64_t N, uint64_t seedBase, 
             eOut->emitted += r.emitted; eOut->absorbed += r.absorbed; eOut->sensor += r.sensor;
             eOut->escaped += r.escaped; eOut->residual += r.residual;
         }
-    if (stats3) {
-        stats3[0] = stats3[1] = stats3[2] = 0;
-        for (auto& c : counts) { stats3[0] += c.segments; stats3[1] += c.connections; stats3[2] += c.splats; }
+    if (stats7) {
+        for (int i = 0; i < 7; ++i) stats7[i] = 0;
+        for (auto& c : counts) {
+            stats7[0] += c.segments; stats7[1] += c.connections; stats7[2] += c.splats;
+            stats7[3] += (uint64_t)c.nodeVisits; stats7[4] += (uint64_t)c.leafTests;
+            stats7[5] += (uint64_t)c.anyNodeVisits; stats7[6] += (uint64_t)c.anyLeafTests;
+        }
     }
     return 0;
 }

@@ -30,9 +30,12 @@ double or_scene_total_power(void* s);
 // returns their number.
 int or_scene_mirror_planes(void* s, double* out10, int cap);
 
+// stats7: segments, connections, splats, then the traversal statistics
+// (TraversalStats) of the segments' closest hits (node visits, leaf tests) and of the
+// camera connections' any hits (node visits, leaf tests).
 int or_render(void* s, const gp_camera_desc* cam, int64_t N, uint64_t seedBase, uint64_t salt,
               int heroC, int maxBounce, int rrDepth, int nThreads, double* xyz, double* hits,
-              gp_energy* eOut, uint64_t* stats3);
+              gp_energy* eOut, uint64_t* stats7);
 
 int or_closest_hit(void* s, const double* rays, int64_t n, double tmin, int32_t* prim,
                    double* t, int64_t* nodeVisits, int64_t* leafTests);

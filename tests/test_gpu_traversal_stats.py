@@ -1,0 +1,63 @@
+"""Traversal statistics of a trace (GP_TRACE_TRAVERSAL_STATS; the reference's
+TraversalStats / -bvhstats, src/bvh.h:79, src/main.cpp:2599), on the device.
+
+The counting kernel variants must leave the film and the path statistics unchanged bit
+for bit; the counts are a function of the traced rays only, so they repeat exactly run
+to run and across queue capacities; every query visits at least the root node. Against
+the oracle's counts over the reference's binary tree on the same photons: a four-wide
+node visit covers up to three binary inner-node visits, so the four-wide count lies
+between a quarter of the binary count and the binary count, and both trees stop at
+leaves of at most four triangles, so the triangle tests agree within a factor of two."""
+import numpy as np
+import pytest
+
+import goodphotons_amd as gp
+from goodphotons_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("c0", lambda: scenes.cornell_c0(64), 1 << 17, 8),
+         ("c1", lambda: scenes.caustics_c1(64), 1 << 17, 16)]
+
+
+def _run(gs, sd, n, depth, stats):
+    f = gp.Film(sd.camera.res_x, sd.camera.res_y, max_photons=n, lib=gs.L)
+    try:
+        e, st = gp.trace(gs, sd.camera, f, n, hero_c=4, max_depth=depth, rr_depth=sd.rr_depth,
+                         traversal_stats=stats)
+        return f.raw()[0], st.as_dict()
+    finally:
+        f.close()
+
+
+@pytest.mark.parametrize("name,make,n,depth", CASES, ids=[c[0] for c in CASES])
+def test_traversal_stats(gplib, orc, name, make, n, depth):
+    sd = make()
+    gs = gp.GpuScene(sd, gplib)
+    w0, s0 = _run(gs, sd, n, depth, False)
+    w1, s1 = _run(gs, sd, n, depth, True)
+    keys = ("node_visits", "tri_tests", "shadow_node_visits", "shadow_tri_tests")
+    print(f"[trav] {name}: " + " ".join(f"{k}={s1[k]}" for k in ("segments", "connections",
+                                                                  "direct_connections") + keys), flush=True)
+    assert np.array_equal(w0, w1)
+    for k in ("photons", "segments", "connections", "splats", "direct_connections"):
+        assert s0[k] == s1[k], k
+    assert all(s0[k] == 0 for k in keys)
+    assert s1["node_visits"] >= s1["segments"] and s1["shadow_node_visits"] >= s1["connections"]
+    assert 0 < s1["direct_connections"] <= s1["photons"] < s1["connections"]
+    # deterministic: the same counts with another queue capacity (other scheduling)
+    old = gplib.tuning()
+    try:
+        gplib.set_tuning(queue_capacity=1 << 15)
+        _, s2 = _run(gs, sd, n, depth, True)
+    finally:
+        gplib.set_tuning(**{k: getattr(old, k) for k, _ in old._fields_})
+    assert all(s2[k] == s1[k] for k in keys)
+    # against the reference's binary tree on the same photons (the oracle's counters)
+    _, _, _, c = orc.render(orc.scene(sd), sd.camera, n, hero_c=4, max_bounce=depth, rr_depth=sd.rr_depth,
+                            threads=8)
+    print(f"[trav] {name} oracle (binary tree): node_visits={c['node_visits']} leaf_tests={c['leaf_tests']} "
+          f"any_node_visits={c['any_node_visits']} any_leaf_tests={c['any_leaf_tests']}", flush=True)
+    assert c["node_visits"] / 4 < s1["node_visits"] < c["node_visits"]
+    assert 0.5 < s1["tri_tests"] / c["leaf_tests"] < 2.0
+    assert c["any_node_visits"] / 4 < s1["shadow_node_visits"] < c["any_node_visits"]
