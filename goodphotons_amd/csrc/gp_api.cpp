@@ -443,17 +443,18 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     std::lock_guard<std::mutex> lock(E.mu);
     if (E.owner)
         return fail(GP_ERR_STATE, "a server has jobs in flight on this device; step it until they finish first");
-    // the statistics variants run for this call only (plain launches, no graph)
-    struct StatsScope {
-        Engine& E;
-        ~StatsScope() { E.q.stats = 0; }
-    } statsScope{E};
-    E.q.stats = (p->flags & GP_TRACE_TRAVERSAL_STATS) ? 1 : 0;
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     ++E.epoch;
     const int lds = E.tun.lds_stack;
     if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, s->stackNeed, n_cams, s->nMirrors > 0)) != GP_OK) return st;
     E.mirrorActive = s->nMirrors > 0;
+    // the statistics variants run for this call only (plain launches, no graph); set
+    // after ensure(), which may rebuild the queue descriptor
+    struct StatsScope {
+        Engine& E;
+        ~StatsScope() { E.q.stats = 0; }
+    } statsScope{E};
+    E.q.stats = (p->flags & GP_TRACE_TRAVERSAL_STATS) ? 1 : 0;
     hipStream_t strm = device().stream;
 
     gp_trace_stats acc{};

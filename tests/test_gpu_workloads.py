@@ -64,7 +64,8 @@ def test_c3_all_frames_in_one_server_run(gplib, orc):
 def test_c4_poisson_serving_alone_equals_mix(gplib):
     sc = workloads.c4_scenes(gplib, full=True)
     print("[c4] scenes " + ", ".join(f"{n}:{sd.n_tris}" for n, sd, _ in sc), flush=True)
-    assert [sd.n_tris for n, sd, _ in sc[4:]] == [100_000, 500_000, 1_000_000, 2_000_000]
+    # (each soup scene adds its light's two triangles)
+    assert [sd.n_tris - 2 for n, sd, _ in sc[4:]] == [100_000, 500_000, 1_000_000, 2_000_000]
     jobs = workloads.c4_trace(12, lam=20.0, seed=4)
     # one 1024^2 film (32 MiB) plus tables fits, two do not
     budget = (1024 * 1024 * 32) + (8 << 20)
