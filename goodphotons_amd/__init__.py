@@ -65,6 +65,7 @@ class Library:
                                               C.POINTER(C.c_int64), C.POINTER(abi.Energy)]
         L.gp_film_tonemap.argtypes = [vp, C.c_int64, C.c_double, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint8), C.POINTER(C.c_double)]
+        L.gp_film_tonemap_noalloc.argtypes = L.gp_film_tonemap.argtypes
         L.gp_trace.argtypes = [vp, C.POINTER(abi.CameraDesc), vp, C.POINTER(abi.TraceParams),
                                C.POINTER(abi.Energy), C.POINTER(abi.TraceStats)]
         L.gp_trace_shared.argtypes = [vp, C.c_int32, C.POINTER(abi.CameraDesc), C.POINTER(vp),
@@ -237,15 +238,17 @@ class Film:
                                                   out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
-    def tonemap(self, n_photons, exposure_comp=0.0, absolute=False, anchor=None):
+    def tonemap(self, n_photons, exposure_comp=0.0, absolute=False, anchor=None, no_alloc=False):
         """8-bit sRGB (res_y, res_x, 3), row 0 = top, with the p99 auto exposure.
-        Returns (image, exposure, anchor); pass the returned anchor back to lock it."""
+        Returns (image, exposure, anchor); pass the returned anchor back to lock it.
+        no_alloc=True uses gp_film_tonemap_noalloc (no device allocation; films up to
+        2048 x 2048)."""
         img = np.zeros((self.ry, self.rx, 3), np.uint8)
         ex = C.c_double(0)
         anc = C.c_double(0.0 if anchor is None else float(anchor))
-        self.L.check(self.L.lib.gp_film_tonemap(self.h, int(n_photons), float(exposure_comp), int(bool(absolute)),
-                                                C.byref(anc), img.ctypes.data_as(C.POINTER(C.c_uint8)),
-                                                C.byref(ex)))
+        fn = self.L.lib.gp_film_tonemap_noalloc if no_alloc else self.L.lib.gp_film_tonemap
+        self.L.check(fn(self.h, int(n_photons), float(exposure_comp), int(bool(absolute)), C.byref(anc),
+                        img.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(ex)))
         return img, ex.value, anc.value
 
     def close(self):

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
 
 #include "gp_runtime.hpp"
 
@@ -159,10 +161,54 @@ __global__ void k_rgb8(const long long* __restrict__ film, int W, int H, double 
     }
 }
 
+// Starts a select: the rank sought, empty prefix, zeroed histogram (a kernel rather
+// than a host copy and memset, so the allocation-free entry point records only
+// kernels on the stream).
+__global__ void __launch_bounds__(256) k_select_init(SelectState* st, unsigned* hist, unsigned long long k) {
+    hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *st = SelectState{0, 0, k, 0};
+}
+
 struct DevMem {
     void* p = nullptr;
     ~DevMem() { if (p) hipFree(p); }
 };
+
+// Scratch memory of the allocation-free tone map (gp_film_tonemap_noalloc): the
+// select state, histogram and exposure, then the anchor keys (8 B per pixel) and the
+// 8-bit image (3 B per pixel) of a film of at most kNoAllocMaxPixels pixels. Each GPU
+// holds its own copy for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress returns on that device (noAllocScratch).
+constexpr size_t kNoAllocMaxPixels = size_t(1) << 22;   // 2048 x 2048
+constexpr size_t kNoAllocHead = 4096;                   // SelectState, 256 bins, 2 doubles
+constexpr size_t kNoAllocKeys = kNoAllocMaxPixels * sizeof(unsigned long long);
+constexpr size_t kNoAllocBytes = kNoAllocHead + kNoAllocKeys + kNoAllocMaxPixels * 3;
+static_assert(sizeof(SelectState) + 256 * sizeof(unsigned) + 2 * sizeof(double) <= kNoAllocHead, "head");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex gScratchMu;
+void* gScratchPtr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device
+// (with that device current) and kept.
+gp_status noAllocScratch(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(GP_ERR_NO_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    if (!gScratchPtr[dev]) {
+        int cur = -1;
+        GP_HIP(hipGetDevice(&cur));
+        if (cur != dev) GP_HIP(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(GP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        gScratchPtr[dev] = p;
+    }
+    *out = (unsigned char*)gScratchPtr[dev];
+    return GP_OK;
+}
 
 gp_status filmNorm(const gp_film* f, int64_t nPhotons, double& inv, double& norm) {
     if (!f) return fail(GP_ERR_INVALID, "null film");
@@ -240,6 +286,54 @@ gp_status gp_film_tonemap(gp_film* film, int64_t n_photons, double exposure_comp
     GP_HIP(hipGetLastError());
     double e[2];
     GP_HIP(hipMemcpyAsync(rgb8, img.p, npix * 3, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipMemcpyAsync(e, expo, sizeof e, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (anchor && !absolute && !(*anchor > 0.0)) *anchor = e[0];
+    if (exposure_out) *exposure_out = e[1];
+    return GP_OK;
+}
+
+gp_status gp_film_tonemap_noalloc(gp_film* film, int64_t n_photons, double exposure_comp, int32_t absolute,
+                                  double* anchor, uint8_t* rgb8, double* exposure_out) {
+    double inv, norm;
+    gp_status st = filmNorm(film, n_photons, inv, norm);
+    if (st != GP_OK) return st;
+    if (!rgb8) return fail(GP_ERR_INVALID, "null output");
+    if ((st = requireDevice()) != GP_OK) return st;
+    const int W = film->rx, H = film->ry;
+    const size_t npix = (size_t)W * H;
+    if (npix > kNoAllocMaxPixels)
+        return fail(GP_ERR_CAPACITY, "gp_film_tonemap_noalloc: film of " + std::to_string(npix) +
+                                         " pixels exceeds its scratch limit of 4194304 (2048 x 2048)");
+    unsigned char* base = nullptr;
+    if ((st = noAllocScratch(device().id, &base)) != GP_OK) return st;
+    SelectState* dst = (SelectState*)base;
+    unsigned* hist = (unsigned*)(base + sizeof(SelectState));
+    double* expo = (double*)((char*)hist + 256 * sizeof(unsigned));
+    unsigned long long* keys = (unsigned long long*)(base + kNoAllocHead);
+    unsigned char* img = base + kNoAllocHead + kNoAllocKeys;
+    hipStream_t s = device().stream;
+    const bool select = !absolute && !(anchor && *anchor > 0.0);
+    // the previous call's state is still in scratch: start the select afresh
+    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, dst, hist,
+                       (unsigned long long)(size_t)(0.99 * (double)(npix - 1)));
+    if (select) {
+        hipLaunchKernelGGL(k_linear, dim3(gridFor(npix)), dim3(256), 0, s, (const long long*)film->d, npix,
+                           inv, norm, (float*)nullptr, keys);
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 56 - 8 * pass;
+            hipLaunchKernelGGL(k_select_hist, dim3(gridFor(npix)), dim3(256), 0, s,
+                               (const unsigned long long*)keys, npix, (const SelectState*)dst, shift, hist);
+            hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(256), 0, s, dst, shift, hist);
+        }
+    }
+    hipLaunchKernelGGL(k_exposure, dim3(1), dim3(64), 0, s, (const SelectState*)dst,
+                       anchor ? *anchor : 0.0, exposure_comp, (int)absolute, expo);
+    hipLaunchKernelGGL(k_rgb8, dim3(gridFor(npix)), dim3(256), 0, s, (const long long*)film->d, W, H, inv,
+                       norm, (const double*)expo, img);
+    GP_HIP(hipGetLastError());
+    double e[2];
+    GP_HIP(hipMemcpyAsync(rgb8, img, npix * 3, hipMemcpyDeviceToHost, s));
     GP_HIP(hipMemcpyAsync(e, expo, sizeof e, hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
     if (anchor && !absolute && !(*anchor > 0.0)) *anchor = e[0];

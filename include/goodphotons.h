@@ -306,6 +306,16 @@ gp_status gp_film_to_linear(gp_film* film, int64_t n_photons, float* rgb);
  * back. exposure_out (optional) receives the exposure used. */
 gp_status gp_film_tonemap(gp_film* film, int64_t n_photons, double exposure_comp, int32_t absolute,
                           double* anchor, uint8_t* rgb8, double* exposure_out);
+/* gp_film_tonemap without any device allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and result, bit for
+ * bit; its working memory is a fixed scratch array in the library (44 MiB per GPU,
+ * held from the library's load until the process ends), so films are limited to
+ * 4194304 pixels (2048 x 2048; larger ones return GP_ERR_CAPACITY). All calls on one
+ * GPU share that array: calls that overlap on one GPU (from several host threads)
+ * race and must be serialised by the caller. */
+gp_status gp_film_tonemap_noalloc(gp_film* film, int64_t n_photons, double exposure_comp, int32_t absolute,
+                                  double* anchor, uint8_t* rgb8, double* exposure_out);
 
 /* ---- tracing ------------------------------------------------------------ */
 /* Trace params->n_photons photons of `scene` through `cam` into `film`

@@ -26,6 +26,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline __attribute__((always_inline))
+#define __align__(n) __attribute__((aligned(n)))
 #define __launch_bounds__(...)
 #define __shared__ static
 
@@ -133,6 +134,8 @@ struct hipDeviceProp_t {
 };
 hipError_t hipGetDeviceCount(int* n);
 hipError_t hipSetDevice(int d);
+hipError_t hipGetDevice(int* d);
+hipError_t hipGetSymbolAddress(void** p, const void* symbol);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int d);
 hipError_t hipMemGetInfo(size_t* f, size_t* t);
 const char* hipGetErrorString(hipError_t e);
