@@ -163,7 +163,7 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sd, res, target_s=12.0):
+def cpu_baseline(sd, res, target_s=15.0):
     """Time the CPU forward path on a bounded sample of the same workload.
 
     With the reference build present (oracle/_ref), the reference itself is timed
@@ -185,7 +185,7 @@ def cpu_baseline(sd, res, target_s=12.0):
     if ref is not None:
         _log("cpu baseline: reference scene + BVH")
     ts = timer.scene(sd) if ref is not None else cs
-    pilot = 1 << 18
+    pilot = 1 << 20   # large enough that per-thread film set-up is a small part of it
     t0 = time.perf_counter()
     timer.render(ts, cam, pilot, seed_base=1 << 41, threads=T, **kw)
     dt = time.perf_counter() - t0

@@ -84,7 +84,7 @@ class Library:
         L.gp_server_take_result.argtypes = [vp, C.c_int64, C.POINTER(abi.JobResult),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gp_server_poll.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]
-        L.gp_server_pending.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.gp_server_pending.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.gp_server_destroy.argtypes = [vp]
         L.gp_server_destroy.restype = None
 
@@ -368,11 +368,13 @@ def render_forward_shared(scene: GpuScene, cams, n, seed_base=0, hero_c=4, max_d
 class Server:
     """Continuous-batching render server over resident scenes (gp_server_*)."""
 
-    def __init__(self, hbm_budget_bytes=16 << 30, max_live_jobs=64, lib: Library | None = None):
+    def __init__(self, hbm_budget_bytes=16 << 30, max_live_jobs=64, lib: Library | None = None,
+                 order_policy=abi.ORDER_SHORTEST_FIRST):
         self.L = lib or load()
         cfg = abi.ServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_live_jobs = int(max_live_jobs)
+        cfg.order_policy = int(order_policy)
         h = C.c_void_p()
         self.L.check(self.L.lib.gp_server_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -400,10 +402,11 @@ class Server:
         self.L.check(self.L.lib.gp_server_poll(self.h, ids, int(capacity), C.byref(n)))
         return [ids[i] for i in range(n.value)]
 
-    def pending(self):
-        q, l = C.c_int32(0), C.c_int32(0)
-        self.L.check(self.L.lib.gp_server_pending(self.h, C.byref(q), C.byref(l)))
-        return q.value, l.value
+    def pending(self, with_bytes=False):
+        """(queued, live) jobs, and with_bytes the footprint the live ones hold."""
+        q, l, b = C.c_int32(0), C.c_int32(0), C.c_int64(0)
+        self.L.check(self.L.lib.gp_server_pending(self.h, C.byref(q), C.byref(l), C.byref(b)))
+        return (q.value, l.value, b.value) if with_bytes else (q.value, l.value)
 
     def take(self, job_id, res_x, res_y, film=True):
         r = abi.JobResult()

@@ -8,7 +8,9 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-SPEC_CONST, SPEC_STEP, SPEC_GAUSS, SPEC_BLACKBODY, SPEC_SELLMEIER, SPEC_CAUCHY = range(6)
+SPEC_CONST, SPEC_STEP, SPEC_GAUSS, SPEC_BLACKBODY, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_SAMPLED = range(7)
+BVH_HOST_SAH, BVH_DEVICE_LBVH, BVH_PREBUILT = 0, 1, 2
+ORDER_SHORTEST_FIRST, ORDER_ADMISSION = 0, 1
 MAT_DIFFUSE, MAT_DIELECTRIC, MAT_MIRROR, MAT_GLOSSY = 0, 1, 2, 4
 
 STATUS_NAMES = {
@@ -19,7 +21,7 @@ STATUS_NAMES = {
 
 
 class Spectrum(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("p", C.c_double * 6),
+    _fields_ = [("kind", C.c_int32), ("table", C.c_int32), ("p", C.c_double * 6),
                 ("band_lo", C.c_double), ("band_hi", C.c_double)]
 
 
@@ -34,10 +36,18 @@ class Emitter(C.Structure):
                 ("reserved", C.c_int32), ("beam_dir", C.c_double * 3), ("spd", Spectrum)]
 
 
+class BvhNode(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("left", C.c_int32), ("right", C.c_int32),
+                ("first", C.c_int32), ("count", C.c_int32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("n_tris", C.c_int64), ("n_materials", C.c_int32), ("n_emitters", C.c_int32),
                 ("verts", C.POINTER(C.c_float)), ("mat_ids", C.POINTER(C.c_int32)),
-                ("materials", C.POINTER(Material)), ("emitters", C.POINTER(Emitter))]
+                ("materials", C.POINTER(Material)), ("emitters", C.POINTER(Emitter)),
+                ("spectrum_samples", C.POINTER(C.c_double)), ("n_spectrum_samples", C.c_int64),
+                ("bvh_nodes", C.POINTER(BvhNode)), ("n_bvh_nodes", C.c_int64),
+                ("bvh_prim_index", C.POINTER(C.c_int32)), ("n_bvh_prim_index", C.c_int64)]
 
 
 class CameraDesc(C.Structure):
@@ -98,7 +108,7 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_live_jobs", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("order_policy", C.c_int32)]
 
 
 class JobDesc(C.Structure):

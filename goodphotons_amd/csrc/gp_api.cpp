@@ -20,10 +20,23 @@ namespace {
 
 // Host evaluation of a descriptor spectrum in double: the same closed forms the
 // reference's Spectrum closures compute (src/spectrum.h:80-143), used for the
-// emitter wavelength CDFs.
-double specEvalHost(const gp_spectrum& s, double w) {
+// emitter wavelength CDFs. A sampled spectrum interpolates its samples in `pool`
+// linearly, clamped to the end samples (the reference GPU backend's specLookup,
+// src/render_cuda.cu:1641-1649).
+double specEvalHost(const gp_spectrum& s, double w, const double* pool) {
     if (s.band_hi > s.band_lo && !(w >= s.band_lo && w <= s.band_hi)) return 0.0;
     switch (s.kind) {
+        case GP_SPEC_SAMPLED: {
+            const double* tab = pool + s.table;
+            const int n = (int)s.p[2];
+            // (multiplied before the division: exact at the sample wavelengths)
+            const double f = (w - s.p[0]) * (double)(n - 1) / (s.p[1] - s.p[0]);
+            if (!(f > 0.0)) return tab[0];
+            if (f >= (double)(n - 1)) return tab[n - 1];
+            const int i = (int)f;
+            const double fr = f - (double)i;
+            return tab[i] * (1.0 - fr) + tab[i + 1] * fr;
+        }
         case GP_SPEC_CONST: return s.p[0];
         case GP_SPEC_STEP: return (w < s.p[2]) ? s.p[0] : s.p[1];
         case GP_SPEC_GAUSS: {
@@ -52,12 +65,12 @@ double specEvalHost(const gp_spectrum& s, double w) {
 }
 
 // 1-nm emission CDF over [360, 830) (EmissionSampler::build, src/spectrum.h:253-265).
-double buildCdf(const gp_spectrum& spd, double* cdf) {
+double buildCdf(const gp_spectrum& spd, double* cdf, const double* pool) {
     double acc = 0.0;
     cdf[0] = 0.0;
     for (int i = 0; i < kLambdaBins; ++i) {
         double w = 360.0 + (i + 0.5) * 1.0;
-        acc += std::max(0.0, specEvalHost(spd, w)) * 1.0;
+        acc += std::max(0.0, specEvalHost(spd, w, pool)) * 1.0;
         cdf[i + 1] = acc;
     }
     if (acc > 0)
@@ -65,11 +78,19 @@ double buildCdf(const gp_spectrum& spd, double* cdf) {
     return acc;
 }
 
-DevSpectrum toDev(const gp_spectrum& s) {
+// `dPool`: the scene's device copy (float) of gp_scene_desc.spectrum_samples.
+DevSpectrum toDev(const gp_spectrum& s, const float* dPool) {
     DevSpectrum d;
     std::memset(&d, 0, sizeof d);
     d.kind = s.kind;
     for (int i = 0; i < 6; ++i) d.p[i] = (float)s.p[i];
+    if (s.kind == GP_SPEC_SAMPLED) {
+        d.n = (int32_t)s.p[2];
+        d.tab = dPool + s.table;
+        d.p[0] = (float)s.p[0];
+        d.p[1] = (float)((double)(d.n - 1) / (s.p[1] - s.p[0]));
+        d.p[2] = 0.0f;
+    }
     d.band_lo = (float)s.band_lo;
     d.band_hi = (float)s.band_hi;
     return d;
@@ -77,12 +98,22 @@ DevSpectrum toDev(const gp_spectrum& s) {
 
 bool finite3(const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
-gp_status validateSpectrum(const gp_spectrum& s, const char* what) {
-    if (s.kind < GP_SPEC_CONST || s.kind > GP_SPEC_CAUCHY)
+gp_status validateSpectrum(const gp_spectrum& s, const char* what, const gp_scene_desc* d) {
+    if (s.kind < GP_SPEC_CONST || s.kind > GP_SPEC_SAMPLED)
         return fail(GP_ERR_UNSUPPORTED, std::string(what) + ": unknown spectrum kind");
     for (double v : s.p)
         if (!std::isfinite(v)) return fail(GP_ERR_INVALID, std::string(what) + ": non-finite spectrum parameter");
     if (s.kind == GP_SPEC_GAUSS && !(s.p[3] != 0.0)) return fail(GP_ERR_INVALID, "gaussian spectrum with zero width");
+    if (s.kind == GP_SPEC_SAMPLED) {
+        const double n = s.p[2];
+        if (!(n >= 2.0 && n <= 1e6 && n == std::floor(n)) || !(s.p[1] > s.p[0]))
+            return fail(GP_ERR_INVALID, std::string(what) + ": sampled spectrum needs >= 2 samples over lo < hi");
+        if (!d->spectrum_samples || s.table < 0 || (int64_t)s.table + (int64_t)n > d->n_spectrum_samples)
+            return fail(GP_ERR_INVALID, std::string(what) + ": sampled spectrum outside spectrum_samples");
+        for (int64_t k = 0; k < (int64_t)n; ++k)
+            if (!std::isfinite(d->spectrum_samples[s.table + k]))
+                return fail(GP_ERR_INVALID, std::string(what) + ": non-finite spectrum sample");
+    }
     return GP_OK;
 }
 
@@ -97,7 +128,7 @@ struct DevBuf {
 using namespace gp;
 
 gp_scene::~gp_scene() {
-    void* ptrs[] = {dNodes, dTris, dTrin, dMats, dEms, dCdfs, dEmCdf, dGuides, dMirrors};
+    void* ptrs[] = {dNodes, dTris, dTrin, dMats, dEms, dCdfs, dEmCdf, dGuides, dMirrors, dSpec};
     for (void* p : ptrs)
         if (p) hipFree(p);
 }
@@ -161,8 +192,8 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
             m.type != GP_MAT_GLOSSY)
             return fail(GP_ERR_UNSUPPORTED, "material type outside {Diffuse, Dielectric, Mirror, Glossy}");
         gp_status st;
-        if ((st = validateSpectrum(m.reflect, "reflect")) != GP_OK) return st;
-        if ((st = validateSpectrum(m.ior, "ior")) != GP_OK) return st;
+        if ((st = validateSpectrum(m.reflect, "reflect", d)) != GP_OK) return st;
+        if ((st = validateSpectrum(m.ior, "ior", d)) != GP_OK) return st;
         if (!std::isfinite(m.roughness)) return fail(GP_ERR_INVALID, "non-finite roughness");
     }
     for (int64_t i = 0; i < d->n_tris; ++i) {
@@ -175,11 +206,16 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         const gp_emitter& e = d->emitters[i];
         if (!finite3(e.origin) || !finite3(e.u) || !finite3(e.v) || !finite3(e.normal) || !(e.area > 0.0))
             return fail(GP_ERR_INVALID, "bad emitter geometry");
-        gp_status st = validateSpectrum(e.spd, "emitter spd");
+        gp_status st = validateSpectrum(e.spd, "emitter spd", d);
         if (st != GP_OK) return st;
     }
-    gp_status st = requireDevice();
-    if (st != GP_OK) return st;
+    if (builder == GP_BVH_PREBUILT && (!d->bvh_nodes || d->n_bvh_nodes <= 0 || !d->bvh_prim_index ||
+                                       d->n_bvh_prim_index <= 0))
+        return fail(GP_ERR_INVALID, "GP_BVH_PREBUILT needs bvh_nodes and bvh_prim_index");
+    gp_status st;
+    // the device builder needs the device now; the host trees are built (and a
+    // caller's tree checked) first
+    if (builder == GP_BVH_DEVICE_LBVH && (st = requireDevice()) != GP_OK) return st;
 
     std::unique_ptr<gp_scene> s(new gp_scene());
     s->nTris = d->n_tris;
@@ -235,7 +271,11 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         auto t0 = std::chrono::steady_clock::now();
         BvhResult bvh;
         try {
-            buildBvh(d->verts, d->n_tris, 8, bvh);
+            if (builder == GP_BVH_PREBUILT)
+                importBvh(d->verts, d->n_tris, d->bvh_nodes, d->n_bvh_nodes, d->bvh_prim_index,
+                          d->n_bvh_prim_index, bvh);
+            else
+                buildBvh(d->verts, d->n_tris, 8, bvh);
         } catch (const std::exception& ex) {
             return fail(GP_ERR_INVALID, ex.what());
         }
@@ -257,37 +297,8 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         collapse4(s->pairs, quads, depth4);
         s->nNodes = (int64_t)(quads.size() / 32);
         s->stackNeed = 3 * depth4;
+        if ((st = requireDevice()) != GP_OK) return st;
     }
-
-    std::vector<DevMaterial> mats((size_t)d->n_materials);
-    for (int i = 0; i < d->n_materials; ++i) {
-        DevMaterial& m = mats[(size_t)i];
-        std::memset(&m, 0, sizeof m);
-        m.type = d->materials[i].type;
-        m.roughness = (float)d->materials[i].roughness;
-        m.reflect = toDev(d->materials[i].reflect);
-        m.ior = toDev(d->materials[i].ior);
-    }
-    const int nEm = d->n_emitters;
-    std::vector<double> cdfs((size_t)nEm * (kLambdaBins + 1));
-    std::vector<DevEmitter> ems((size_t)nEm);
-    std::vector<double> powers((size_t)nEm);
-    double total = 0.0;
-    std::vector<double> emCdf((size_t)nEm);
-    for (int i = 0; i < nEm; ++i) {
-        const gp_emitter& e = d->emitters[i];
-        double integral = buildCdf(e.spd, &cdfs[(size_t)i * (kLambdaBins + 1)]);
-        powers[(size_t)i] = integral * e.area * 3.141592653589793;
-        total += powers[(size_t)i];
-        emCdf[(size_t)i] = total;
-        s->emitQuads.push_back({e.origin[0], e.origin[1], e.origin[2], e.u[0], e.u[1], e.u[2],
-                                e.v[0], e.v[1], e.v[2], e.normal[0], e.normal[1], e.normal[2]});
-    }
-    if (!(total > 0.0)) return fail(GP_ERR_INVALID, "emitters carry no power");
-    for (auto& c : emCdf) c /= total;
-    s->totalPower = total;
-    s->nEm = nEm;
-    s->basePower = (nEm == 1) ? powers[0] : total;
 
     auto upload = [&](const void* src, size_t bytes, void** dst) -> gp_status {
         hipError_t e = hipMalloc(dst, bytes);
@@ -300,6 +311,41 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         s->deviceBytes += (int64_t)bytes;
         return GP_OK;
     };
+    if (d->spectrum_samples && d->n_spectrum_samples > 0) {
+        std::vector<float> pool((size_t)d->n_spectrum_samples);
+        for (size_t k = 0; k < pool.size(); ++k) pool[k] = (float)d->spectrum_samples[k];
+        if ((st = upload(pool.data(), pool.size() * sizeof(float), (void**)&s->dSpec)) != GP_OK) return st;
+    }
+    std::vector<DevMaterial> mats((size_t)d->n_materials);
+    for (int i = 0; i < d->n_materials; ++i) {
+        DevMaterial& m = mats[(size_t)i];
+        std::memset(&m, 0, sizeof m);
+        m.type = d->materials[i].type;
+        m.roughness = (float)d->materials[i].roughness;
+        m.reflect = toDev(d->materials[i].reflect, s->dSpec);
+        m.ior = toDev(d->materials[i].ior, s->dSpec);
+    }
+    const int nEm = d->n_emitters;
+    std::vector<double> cdfs((size_t)nEm * (kLambdaBins + 1));
+    std::vector<DevEmitter> ems((size_t)nEm);
+    std::vector<double> powers((size_t)nEm);
+    double total = 0.0;
+    std::vector<double> emCdf((size_t)nEm);
+    for (int i = 0; i < nEm; ++i) {
+        const gp_emitter& e = d->emitters[i];
+        double integral = buildCdf(e.spd, &cdfs[(size_t)i * (kLambdaBins + 1)], d->spectrum_samples);
+        powers[(size_t)i] = integral * e.area * 3.141592653589793;
+        total += powers[(size_t)i];
+        emCdf[(size_t)i] = total;
+        s->emitQuads.push_back({e.origin[0], e.origin[1], e.origin[2], e.u[0], e.u[1], e.u[2],
+                                e.v[0], e.v[1], e.v[2], e.normal[0], e.normal[1], e.normal[2]});
+    }
+    if (!(total > 0.0)) return fail(GP_ERR_INVALID, "emitters carry no power");
+    for (auto& c : emCdf) c /= total;
+    s->totalPower = total;
+    s->nEm = nEm;
+    s->basePower = (nEm == 1) ? powers[0] : total;
+
     if (builder != GP_BVH_DEVICE_LBVH) {
         if ((st = upload(quads.data(), quads.size() * sizeof(float), (void**)&s->dNodes)) != GP_OK) return st;
         if ((st = upload(tris.data(), tris.size() * sizeof(float4), (void**)&s->dTris)) != GP_OK) return st;
@@ -351,7 +397,8 @@ gp_status gp_scene_create(const gp_scene_desc* d, gp_scene** out) {
 }
 
 gp_status gp_scene_create_ex(const gp_scene_desc* d, int32_t builder, gp_scene** out) {
-    if (builder != GP_BVH_HOST_SAH && builder != GP_BVH_DEVICE_LBVH) return fail(GP_ERR_INVALID, "unknown BVH builder");
+    if (builder != GP_BVH_HOST_SAH && builder != GP_BVH_DEVICE_LBVH && builder != GP_BVH_PREBUILT)
+        return fail(GP_ERR_INVALID, "unknown BVH builder");
     return createScene(d, builder, out);
 }
 

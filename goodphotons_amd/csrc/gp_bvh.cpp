@@ -150,35 +150,10 @@ inline float up(double v) {
     return ((double)f < v) ? std::nextafter(f, FLT_MAX) : f;
 }
 
-}  // namespace
-
-void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out) {
-    if (n <= 0) throw std::runtime_error("scene has no triangles");
-    if (n >= (int64_t(1) << 28)) throw std::runtime_error("more than 2^28 triangles");
-    std::vector<Prim> P((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-        const float* v = verts + 9 * i;
-        Prim& p = P[(size_t)i];
-        for (int a = 0; a < 3; ++a) {
-            double lo = std::min({(double)v[a], (double)v[3 + a], (double)v[6 + a]}) - kPad;
-            double hi = std::max({(double)v[a], (double)v[3 + a], (double)v[6 + a]}) + kPad;
-            p.lo[a] = down(lo);
-            p.hi[a] = up(hi);
-            p.c[a] = (float)(0.5 * (lo + hi));
-        }
-    }
-    std::vector<int> idx((size_t)n);
-    for (int64_t i = 0; i < n; ++i) idx[(size_t)i] = (int)i;
-    std::vector<BNode> nodes((size_t)(2 * n));
-    int parDepth = 0;
-    while ((1 << parDepth) < std::max(1, threads) && parDepth < 6) ++parDepth;
-    Builder bld{P, idx, nodes, parDepth};
-    bld.build(0, 0, (int)n, 0);
-
-    // Child-pair layout in depth-first order. Each pair node holds its children's
-    // boxes; a link >= 0 is a pair node, < 0 a leaf code.
+// Child-pair layout in depth-first order from a binary tree (node 0 the root). Each
+// pair node holds its children's boxes; a link >= 0 is a pair node, < 0 a leaf code.
+void emitPairs(const std::vector<BNode>& nodes, BvhResult& out) {
     out.pairs.clear();
-    out.order = idx;
     out.depth = 0;
     out.leaves = 0;
     std::vector<float>& pr = out.pairs;
@@ -186,8 +161,6 @@ void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out) {
         const BNode& c = nodes[slot];
         return c.count > 0 ? leafCode(c.first, c.count) : -2;   // -2: internal, patched later
     };
-    struct Item { int slot; int pairIdx; int depth; };
-    std::vector<Item> stack;
     auto emitPair = [&](int lslot, int rslot) -> int {
         const int pi = (int)(pr.size() / 16);
         pr.resize(pr.size() + 16);
@@ -231,7 +204,95 @@ void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out) {
         if (R.count == 0) todo.push_back({nd.right, pi, 1, p.depth + 1});
         if (L.count == 0) todo.push_back({nd.left, pi, 0, p.depth + 1});
     }
-    (void)stack;
+}
+
+}  // namespace
+
+void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out) {
+    if (n <= 0) throw std::runtime_error("scene has no triangles");
+    if (n >= (int64_t(1) << 28)) throw std::runtime_error("more than 2^28 triangles");
+    std::vector<Prim> P((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* v = verts + 9 * i;
+        Prim& p = P[(size_t)i];
+        for (int a = 0; a < 3; ++a) {
+            double lo = std::min({(double)v[a], (double)v[3 + a], (double)v[6 + a]}) - kPad;
+            double hi = std::max({(double)v[a], (double)v[3 + a], (double)v[6 + a]}) + kPad;
+            p.lo[a] = down(lo);
+            p.hi[a] = up(hi);
+            p.c[a] = (float)(0.5 * (lo + hi));
+        }
+    }
+    std::vector<int> idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) idx[(size_t)i] = (int)i;
+    std::vector<BNode> nodes((size_t)(2 * n));
+    int parDepth = 0;
+    while ((1 << parDepth) < std::max(1, threads) && parDepth < 6) ++parDepth;
+    Builder bld{P, idx, nodes, parDepth};
+    bld.build(0, 0, (int)n, 0);
+
+    out.order = idx;
+    emitPairs(nodes, out);
+}
+
+void importBvh(const float* verts, int64_t nTris, const gp_bvh_node* in, int64_t nNodes,
+               const int32_t* primIndex, int64_t nIndex, BvhResult& out) {
+    if (!in || nNodes <= 0 || !primIndex || nIndex <= 0) throw std::runtime_error("prebuilt BVH: empty");
+    if (nNodes >= (int64_t(1) << 29) || nIndex >= (int64_t(1) << 28))
+        throw std::runtime_error("prebuilt BVH: too many nodes or primitive references");
+    std::vector<BNode> nodes((size_t)nNodes);
+    std::vector<char> seen((size_t)nNodes, 0);
+    std::vector<int> refs((size_t)nTris, 0);
+    auto inside = [](const gp_bvh_node& c, const gp_bvh_node& p) {
+        for (int a = 0; a < 3; ++a)
+            if (!(c.lo[a] >= p.lo[a] && c.hi[a] <= p.hi[a])) return false;
+        return true;
+    };
+    struct Visit { int64_t node; int depth; };
+    std::vector<Visit> todo{{0, 1}};
+    while (!todo.empty()) {
+        const Visit v = todo.back();
+        todo.pop_back();
+        if (seen[(size_t)v.node]) throw std::runtime_error("prebuilt BVH: a node is reached twice");
+        seen[(size_t)v.node] = 1;
+        if (v.depth > 1024) throw std::runtime_error("prebuilt BVH: deeper than 1024 levels");
+        const gp_bvh_node& g = in[v.node];
+        BNode& b = nodes[(size_t)v.node];
+        for (int a = 0; a < 3; ++a) {
+            if (!(g.lo[a] <= g.hi[a])) throw std::runtime_error("prebuilt BVH: empty or non-finite box");
+            b.lo[a] = down(g.lo[a]);
+            b.hi[a] = up(g.hi[a]);
+        }
+        if (g.count > 0) {
+            if (g.count > kLeafMax) throw std::runtime_error("prebuilt BVH: a leaf holds more than 4 primitives");
+            if (g.first < 0 || (int64_t)g.first + g.count > nIndex)
+                throw std::runtime_error("prebuilt BVH: leaf range outside the primitive index");
+            for (int k = 0; k < g.count; ++k) {
+                const int32_t prim = primIndex[g.first + k];
+                if (prim < 0 || prim >= nTris) throw std::runtime_error("prebuilt BVH: primitive index out of range");
+                const float* t = verts + 9 * (size_t)prim;
+                for (int c = 0; c < 9; ++c)
+                    if (!((double)t[c] >= g.lo[c % 3] && (double)t[c] <= g.hi[c % 3]))
+                        throw std::runtime_error("prebuilt BVH: a leaf box does not contain its triangle");
+                refs[(size_t)prim]++;
+            }
+            b.first = g.first;
+            b.count = g.count;
+        } else {
+            if (g.left < 0 || g.left >= nNodes || g.right < 0 || g.right >= nNodes)
+                throw std::runtime_error("prebuilt BVH: child index out of range");
+            if (!inside(in[g.left], g) || !inside(in[g.right], g))
+                throw std::runtime_error("prebuilt BVH: a child box is not inside its parent's");
+            b.left = g.left;
+            b.right = g.right;
+            todo.push_back({g.right, v.depth + 1});
+            todo.push_back({g.left, v.depth + 1});
+        }
+    }
+    for (int64_t p = 0; p < nTris; ++p)
+        if (refs[(size_t)p] == 0) throw std::runtime_error("prebuilt BVH: a triangle is in no leaf");
+    out.order.assign(primIndex, primIndex + nIndex);
+    emitPairs(nodes, out);
 }
 
 void collapse4(const std::vector<float>& pairs, std::vector<float>& quads, int& depth4) {

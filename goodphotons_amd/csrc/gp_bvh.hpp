@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "goodphotons.h"
+
 namespace gp {
 
 struct BvhResult {
@@ -28,5 +30,13 @@ void collapse4(const std::vector<float>& pairs, std::vector<float>& quads, int& 
 
 // verts: n*9 floats. Throws std::runtime_error on inputs it cannot index.
 void buildBvh(const float* verts, int64_t n, int threads, BvhResult& out);
+
+// The caller's binary tree (the reference's Bvh::nodes / primIdx) in the same
+// child-pair form: boxes rounded outwards to float, leaves of at most 4 primitives.
+// Checked as it is read: every node reached once from node 0, child boxes inside
+// their parent's, every leaf's triangles inside its box, every triangle in some
+// leaf. Throws std::runtime_error (with the reason) on a tree that fails a check.
+void importBvh(const float* verts, int64_t nTris, const gp_bvh_node* nodes, int64_t nNodes,
+               const int32_t* primIndex, int64_t nIndex, BvhResult& out);
 
 }  // namespace gp

@@ -110,6 +110,14 @@ __device__ __forceinline__ float specEval(const DevSpectrum& s, float w) {
             float um = w * 1e-3f;
             return s.p[0] + s.p[1] / (um * um);
         }
+        case 6: {   // sampled: linear interpolation, clamped to the end samples
+            const float f = (w - s.p[0]) * s.p[1];
+            if (!(f > 0.0f)) return s.tab[0];
+            if (f >= (float)(s.n - 1)) return s.tab[s.n - 1];
+            const int i = (int)f;
+            const float fr = f - (float)i;
+            return s.tab[i] * (1.0f - fr) + s.tab[i + 1] * fr;
+        }
         default: return 0.0f;
     }
 }

@@ -27,6 +27,7 @@ struct gp_scene {
     double* dEmCdf = nullptr;
     int32_t* dGuides = nullptr;        // per emitter kCdfGuide + 1 CDF guide entries
     gp::DevMirrorPlane* dMirrors = nullptr;   // flat mirror planes (camera connections through them)
+    float* dSpec = nullptr;            // samples of the scene's sampled spectra
     int nMirrors = 0;
     int nEm = 0;
     double totalPower = 0.0;

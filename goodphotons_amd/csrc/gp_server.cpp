@@ -82,6 +82,7 @@ struct gp_server {
         gp_energy energy{};
         gp_trace_stats stats{};
         int64_t installSeq = 0;                  // first chunk that ran with the job installed
+        unsigned long long remaining = 0;        // photons not yet emitted (last snapshot)
         unsigned long long* accHost = nullptr;   // pinned, kSegs * kAccRow words
         long long* filmHost = nullptr;           // pinned, res * 4 words
         hipEvent_t accEv = nullptr, filmEv = nullptr;
@@ -89,7 +90,7 @@ struct gp_server {
     std::map<int64_t, Job> jobs;
     std::deque<int64_t> waiting;
     std::vector<int64_t> slotJob;   // job id per slot, -1 free
-    std::vector<int64_t> order;     // live job ids in admission order
+    std::vector<int64_t> order;     // live job ids in emission priority order
     int64_t nextId = 1;
     int64_t liveBytes = 0;
     int64_t finished = 0;
@@ -131,6 +132,23 @@ namespace {
 
 int64_t jobFootprint(const gp_job_desc& j) {
     return (int64_t)j.camera.res_x * j.camera.res_y * 32 + (int64_t)sizeof(DevJob) + 4096;
+}
+
+// Emission priority of the live jobs. With GP_ORDER_SHORTEST_FIRST the scheduling
+// kernel fills free path slots for the job with the fewest photons left to emit
+// first, so a small job admitted behind large ones finishes without waiting for them
+// (their photons go into the slots left over); equal counts keep admission order.
+// With GP_ORDER_ADMISSION the order is the admission order. Returns whether the order
+// changed.
+bool reorder(gp_server* srv) {
+    if (srv->cfg.order_policy != GP_ORDER_SHORTEST_FIRST) return false;
+    std::vector<int64_t> o = srv->order;
+    std::stable_sort(o.begin(), o.end(), [&](int64_t a, int64_t b) {
+        return srv->jobs[a].remaining < srv->jobs[b].remaining;
+    });
+    if (o == srv->order) return false;
+    srv->order.swap(o);
+    return true;
 }
 
 gp_status uploadOrder(Engine& E, gp_server* srv, int clearSlot) {
@@ -207,13 +225,17 @@ gp_status admit(Engine& E, gp_server* srv) {
         J.slot = slot;
         J.state = JobState::Live;
         J.installSeq = srv->chunkSeq;
+        J.remaining = (unsigned long long)J.desc.params.n_photons;
         J.startMs = srv->now();
         srv->slotJob[(size_t)slot] = id;
         srv->order.push_back(id);
         srv->liveBytes += J.footprint;
         changed = true;
     }
-    if (changed) return uploadOrder(E, srv, -1);
+    if (changed) {
+        reorder(srv);
+        return uploadOrder(E, srv, -1);
+    }
     return GP_OK;
 }
 
@@ -296,8 +318,11 @@ gp_status processOldest(Engine& E, gp_server* srv) {
             changed = true;
             continue;
         }
+        if (c.seq >= J.installSeq)
+            J.remaining = h.endPhoton[J.slot] > h.nextPhoton[J.slot] ? h.endPhoton[J.slot] - h.nextPhoton[J.slot] : 0;
         ++k;
     }
+    changed = reorder(srv) || changed;
     if (changed) return uploadOrder(E, srv, -1);
     return GP_OK;
 }
@@ -325,6 +350,8 @@ gp_status gp_server_create(const gp_server_config* cfg, gp_server** out) {
     *out = nullptr;
     if (cfg->max_live_jobs < 1 || cfg->max_live_jobs > kMaxJobs) return fail(GP_ERR_INVALID, "max_live_jobs must be 1..256");
     if (cfg->hbm_budget_bytes <= 0) return fail(GP_ERR_INVALID, "hbm_budget_bytes must be positive");
+    if (cfg->order_policy != GP_ORDER_SHORTEST_FIRST && cfg->order_policy != GP_ORDER_ADMISSION)
+        return fail(GP_ERR_INVALID, "unknown order_policy");
     gp_status st = requireDevice();
     if (st != GP_OK) return st;
     std::unique_ptr<gp_server> s(new gp_server());
@@ -496,10 +523,11 @@ gp_status gp_server_poll(gp_server* srv, int64_t* ids, int32_t capacity, int32_t
     return GP_OK;
 }
 
-gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live) {
+gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live, int64_t* liveBytes) {
     if (!srv) return fail(GP_ERR_INVALID, "null server");
     if (queued) *queued = (int32_t)srv->waiting.size();
     if (live) *live = (int32_t)srv->order.size();
+    if (liveBytes) *liveBytes = srv->liveBytes;
     return GP_OK;
 }
 

@@ -49,9 +49,10 @@ inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (c
 
 struct DevSpectrum {
     int32_t kind;
-    float p[6];
+    int32_t n;           // sampled: number of samples
+    const float* tab;    // sampled: the samples (the scene's device table)
+    float p[6];          // sampled: p[0] first sample's wavelength, p[1] samples per nm
     float band_lo, band_hi;
-    int32_t pad[3];
 };  // 48 B
 
 struct DevMaterial {

@@ -16,9 +16,9 @@
 //          one camera: scene and film resident in HBM between batches.
 //
 // The scene types mirror the subset of the reference's Scene the core supports
-// (triangles, Diffuse / Dielectric / Glossy materials with analytic spectra, quad
-// area lights, pinhole camera); a caller holding the reference's own Scene/Camera
-// copies those fields into these structs before calling ForwardSession.
+// (triangles, Diffuse / Dielectric / Mirror / Glossy materials with analytic or sampled
+// spectra, quad area lights, pinhole camera). A caller holding the reference's own
+// Scene/Camera uses ref_adapter.hpp instead, which converts them itself.
 #pragma once
 
 #include <cstdint>
@@ -43,6 +43,7 @@ struct Scene {
     std::vector<Tri> tris;
     std::vector<gp_material> mats;
     std::vector<gp_emitter> emitters;
+    std::vector<double> spectrumSamples;   // samples of GP_SPEC_SAMPLED spectra
 };
 
 struct Camera {
@@ -94,6 +95,8 @@ inline gp_scene_desc describe(const Scene& sc, std::vector<float>& verts, std::v
     d.mat_ids = mats.data();
     d.materials = sc.mats.data();
     d.emitters = sc.emitters.data();
+    d.spectrum_samples = sc.spectrumSamples.empty() ? nullptr : sc.spectrumSamples.data();
+    d.n_spectrum_samples = (int64_t)sc.spectrumSamples.size();
     return d;
 }
 inline gp_camera_desc camera(const Camera& c, int resX, int resY) {

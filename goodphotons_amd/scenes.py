@@ -53,6 +53,7 @@ class SceneData:
     max_depth: int = 16
     rr_depth: int = 0
     hero_c: int = 4
+    spectrum_samples: np.ndarray | None = None   # samples of SPEC_SAMPLED spectra
 
     @property
     def n_tris(self) -> int:
@@ -72,6 +73,10 @@ class SceneData:
         d.mat_ids = self._m.ctypes.data_as(C.POINTER(C.c_int32))
         d.materials = C.cast(self._mats, C.POINTER(abi.Material))
         d.emitters = C.cast(self._ems, C.POINTER(abi.Emitter))
+        if self.spectrum_samples is not None and len(self.spectrum_samples):
+            self._samples = np.ascontiguousarray(self.spectrum_samples, dtype=np.float64)
+            d.spectrum_samples = self._samples.ctypes.data_as(C.POINTER(C.c_double))
+            d.n_spectrum_samples = len(self._samples)
         self._desc = d
         return d
 
@@ -80,7 +85,7 @@ class SceneData:
                      self.camera.fov_y_deg, rx, ry if ry is not None else rx)
         out = SceneData(self.name, self.verts, self.mat_ids, list(self.materials),
                         list(self.emitters), cam, self.n_photons, self.max_depth,
-                        self.rr_depth, self.hero_c)
+                        self.rr_depth, self.hero_c, self.spectrum_samples)
         return out
 
 

@@ -86,12 +86,19 @@ enum {
     GP_SPEC_BLACKBODY = 3,  /* v = p[1] * Planck(T = p[0] K, lam)  (SI, per metre) */
     GP_SPEC_SELLMEIER = 4,  /* n = sqrt(max(1, 1 + sum Bi l^2/(l^2-Ci))), l in um,
                                B = p[0..2], C = p[3..5] (um^2)                      */
-    GP_SPEC_CAUCHY = 5      /* n = p[0] + p[1]/l^2, l in um                         */
+    GP_SPEC_CAUCHY = 5,     /* n = p[0] + p[1]/l^2, l in um                         */
+    GP_SPEC_SAMPLED = 6     /* n = (int)p[2] >= 2 samples at lam_k = p[0] + k (p[1]-p[0])/(n-1),
+                               gp_scene_desc.spectrum_samples[table .. table + n); linear
+                               interpolation between samples, the end samples outside
+                               [p[0], p[1]] (the reference GPU backend's baked tables,
+                               bakeSpec / specLookup, src/render_cuda.cu:12658-12663,
+                               :1641-1649). Any Spectrum function of the reference's scene
+                               reaches the core this way.                          */
 };
 
 typedef struct gp_spectrum {
     int32_t kind;
-    int32_t reserved;
+    int32_t table;           /* GP_SPEC_SAMPLED: first sample in spectrum_samples, else 0 */
     double p[6];
     double band_lo, band_hi;
 } gp_spectrum;
@@ -134,6 +141,13 @@ typedef struct gp_emitter {
 
 /* Triangle-only scene. Global primitive id = index into the triangle list,
  * exactly as the reference numbers prims (tris first, src/scene.h:2005-2044). */
+/* A node of a caller's binary BVH, as the reference's BvhNode (src/bvh.h:70-75). */
+typedef struct gp_bvh_node {
+    double lo[3], hi[3];         /* box                                                */
+    int32_t left, right;         /* inner node (count == 0): children (node indices)   */
+    int32_t first, count;        /* leaf (count 1..4): bvh_prim_index[first .. +count) */
+} gp_bvh_node;
+
 typedef struct gp_scene_desc {
     int64_t n_tris;
     int32_t n_materials;
@@ -142,6 +156,16 @@ typedef struct gp_scene_desc {
     const int32_t* mat_ids;      /* n_tris                                    */
     const gp_material* materials;
     const gp_emitter* emitters;
+    /* Samples of the GP_SPEC_SAMPLED spectra (may be null when there are none). */
+    const double* spectrum_samples;
+    int64_t n_spectrum_samples;
+    /* The caller's own binary BVH over the triangles (the reference's Bvh::nodes and
+     * primIdx, src/bvh.h:82-84; node 0 the root), read by gp_scene_create_ex with
+     * GP_BVH_PREBUILT and ignored otherwise (may be null). */
+    const gp_bvh_node* bvh_nodes;
+    int64_t n_bvh_nodes;
+    const int32_t* bvh_prim_index;
+    int64_t n_bvh_prim_index;
 } gp_scene_desc;
 
 /* Pinhole camera as Camera::lookAt builds it (src/camera.h:110-129). */
@@ -237,9 +261,16 @@ gp_status gp_scene_create(const gp_scene_desc* desc, gp_scene** out);
 /* BVH builders for gp_scene_create_ex. GP_BVH_HOST_SAH is gp_scene_create's: binned
  * SAH on the host (3 axes x 32 bins), the higher-quality tree. GP_BVH_DEVICE_LBVH
  * builds a linear BVH on the device (Morton order, Karras 2012) in milliseconds,
- * for large or per-frame geometry; closest-hit results are the same either way. */
+ * for large or per-frame geometry; closest-hit results are the same either way.
+ * GP_BVH_PREBUILT takes the caller's tree (desc->bvh_nodes, bvh_prim_index), as the
+ * reference hands its Scene::bvh to the device verbatim (src/render_cuda.cu:12764-12772):
+ * no build at all. It is checked as it is read (every node reached once from node 0,
+ * child boxes inside their parent's, every leaf's triangles inside its box, leaves of
+ * at most 4 triangles, every triangle in some leaf) and GP_ERR_INVALID names the first
+ * failed check. */
 #define GP_BVH_HOST_SAH 0
 #define GP_BVH_DEVICE_LBVH 1
+#define GP_BVH_PREBUILT 2
 gp_status gp_scene_create_ex(const gp_scene_desc* desc, int32_t bvh_builder, gp_scene** out);
 gp_status gp_scene_get_info(const gp_scene* scene, gp_scene_info* out);
 /* Flat mirror planes the camera connection goes through, as gp_scene_create
@@ -370,8 +401,15 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam,
 typedef struct gp_server_config {
     int64_t hbm_budget_bytes; /* admission limit for films + queues of live jobs   */
     int32_t max_live_jobs;    /* job slots in the device job table (<= 256)        */
-    int32_t reserved;
+    int32_t order_policy;     /* GP_ORDER_SHORTEST_FIRST (0) or GP_ORDER_ADMISSION */
 } gp_server_config;
+
+/* Which live job's photons fill free path slots first. Admission itself is first come
+ * first admitted either way. SHORTEST_FIRST: the job with the fewest photons left to
+ * emit (a small job admitted behind large ones does not wait for them); ADMISSION:
+ * the order of admission. */
+#define GP_ORDER_SHORTEST_FIRST 0
+#define GP_ORDER_ADMISSION 1
 
 typedef struct gp_job_desc {
     const gp_scene* scene;
@@ -416,7 +454,9 @@ gp_status gp_server_poll(gp_server* srv, int64_t* ids_out, int32_t capacity, int
  * film is released after this call. */
 gp_status gp_server_take_result(gp_server* srv, int64_t job_id, gp_job_result* res,
                                 double* xyz, double* hits);
-gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live);
+/* Jobs waiting for admission, jobs admitted and not yet retired, and the HBM
+ * footprint the admitted ones hold against hbm_budget_bytes (any pointer may be null). */
+gp_status gp_server_pending(gp_server* srv, int32_t* queued, int32_t* live, int64_t* live_bytes);
 void gp_server_destroy(gp_server* srv);
 
 #ifdef __cplusplus

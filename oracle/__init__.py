@@ -4,6 +4,9 @@
   Reference()   the reference renderer's own forward path compiled from its
                 headers (oracle/_ref/libref.so; built only where the reference
                 sources exist, travels as a built .so)
+  RefAdapter()  the drop-in adapter compiled against the reference's headers, on
+                scenes built with the reference's own API, beside the reference's
+                CPU path (oracle/_ref/libref_adapter.so, built the same way)
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
 package, and only as the checker / baseline.
@@ -31,8 +34,8 @@ _u64p = C.POINTER(C.c_uint64)
 
 
 def build(with_ref: bool = True) -> None:
-    target = "all" if with_ref else os.path.join(HERE, "liboracle.so")
-    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile"), target], check=True)
+    targets = ["all", "adapter"] if with_ref else [os.path.join(HERE, "liboracle.so")]
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile"), *targets], check=True)
 
 
 def _ptr(a, t):
@@ -65,13 +68,13 @@ class _CpuRenderer:
         f.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _dp]
         f = getattr(L, p + "mix64"); f.restype = C.c_uint64; f.argtypes = [C.c_uint64]
         f = getattr(L, p + "spectrum_eval"); f.restype = None
-        f.argtypes = [C.POINTER(abi.Spectrum), _dp, C.c_int, _dp]
+        f.argtypes = [C.POINTER(abi.Spectrum), _dp, _dp, C.c_int, _dp]
         f = getattr(L, p + "cie"); f.restype = None; f.argtypes = [_dp, C.c_int, _dp]
         f = getattr(L, p + "cie_y_integral"); f.restype = C.c_double; f.argtypes = []
         f = getattr(L, p + "emission_sample"); f.restype = None
-        f.argtypes = [C.POINTER(abi.Spectrum), _dp, C.c_int, _dp, _dp, _dp]
+        f.argtypes = [C.POINTER(abi.Spectrum), _dp, _dp, C.c_int, _dp, _dp, _dp]
         f = getattr(L, p + "sample_bundle"); f.restype = C.c_int
-        f.argtypes = [C.POINTER(abi.Spectrum), C.c_double, C.c_int, _dp, _dp]
+        f.argtypes = [C.POINTER(abi.Spectrum), _dp, C.c_double, C.c_int, _dp, _dp]
         f = getattr(L, p + "cosine_hemisphere"); f.restype = None
         f.argtypes = [_dp, C.c_uint64, C.c_uint64, C.c_int, _dp]
         f = getattr(L, p + "sample_glossy"); f.restype = None
@@ -111,10 +114,13 @@ class _CpuRenderer:
     def mix64(self, x):
         return int(self._f("mix64")(x))
 
-    def spectrum_eval(self, spec, lam):
+    def spectrum_eval(self, spec, lam, pool=None):
+        """pool: the samples of a GP_SPEC_SAMPLED spectrum's table (spec.table indexes it)."""
         lam = np.ascontiguousarray(lam, np.float64)
         out = np.zeros_like(lam)
-        self._f("spectrum_eval")(C.byref(spec), _ptr(lam, _dp), len(lam), _ptr(out, _dp))
+        pl = None if pool is None else np.ascontiguousarray(pool, np.float64)
+        self._f("spectrum_eval")(C.byref(spec), None if pl is None else _ptr(pl, _dp), _ptr(lam, _dp), len(lam),
+                                 _ptr(out, _dp))
         return out
 
     def cie(self, lam):
@@ -163,19 +169,22 @@ class _CpuRenderer:
                                  C.byref(ea), C.byref(ex))
         return img, ea.value, ex.value, (None if anc is None else anc.value)
 
-    def emission_sample(self, spec, u):
+    def emission_sample(self, spec, u, pool=None):
         u = np.ascontiguousarray(u, np.float64)
         lam = np.zeros_like(u)
         pdf = np.zeros_like(u)
         integral = C.c_double(0)
-        self._f("emission_sample")(C.byref(spec), _ptr(u, _dp), len(u), _ptr(lam, _dp),
-                                   _ptr(pdf, _dp), C.byref(integral))
+        pl = None if pool is None else np.ascontiguousarray(pool, np.float64)
+        self._f("emission_sample")(C.byref(spec), None if pl is None else _ptr(pl, _dp), _ptr(u, _dp), len(u),
+                                   _ptr(lam, _dp), _ptr(pdf, _dp), C.byref(integral))
         return lam, pdf, integral.value
 
-    def sample_bundle(self, spec, u, c):
+    def sample_bundle(self, spec, u, c, pool=None):
         lam = np.zeros(8)
         pdf = np.zeros(8)
-        ok = self._f("sample_bundle")(C.byref(spec), u, c, _ptr(lam, _dp), _ptr(pdf, _dp))
+        pl = None if pool is None else np.ascontiguousarray(pool, np.float64)
+        ok = self._f("sample_bundle")(C.byref(spec), None if pl is None else _ptr(pl, _dp), u, c, _ptr(lam, _dp),
+                                      _ptr(pdf, _dp))
         return bool(ok), lam[:c], pdf[:c]
 
     def cosine_hemisphere(self, nrm, seq, seed, n):
@@ -349,4 +358,91 @@ class Reference(_CpuRenderer):
 
 
 def available(which: str = "oracle") -> bool:
-    return os.path.exists(ORACLE_SO if which == "oracle" else REF_SO)
+    return os.path.exists({"oracle": ORACLE_SO, "ref": REF_SO, "adapter": ADAPTER_SO}[which])
+
+
+ADAPTER_SO = os.path.join(HERE, "_ref", "libref_adapter.so")
+
+
+class AdapterScene:
+    """A scene built with the reference's own API inside oracle/_ref/libref_adapter.so
+    (ref_adapter_check.cpp), seen through the drop-in adapter
+    (goodphotons_amd/host/ref_adapter.hpp). `which`: 0 box, 1 box + glass + glossy +
+    mirror, 10..16 variants outside the supported set."""
+
+    def __init__(self, lib, which):
+        self.L = lib
+        self.which = which
+        self.h = lib.adapter_scene(which)
+
+    def supported(self):
+        buf = C.create_string_buffer(256)
+        ok = self.L.adapter_supported(self.h, buf, 256)
+        return bool(ok), buf.value.decode()
+
+    def desc(self):
+        """The gp_scene_desc the adapter hands over (a view into the adapter's memory)."""
+        p = self.L.adapter_desc(self.h)
+        if not p:
+            raise RuntimeError("scene outside the supported set")
+        return abi.SceneDesc.from_address(p)
+
+    @property
+    def n_tris(self):
+        return int(self.L.adapter_n_tris(self.h))
+
+    def close(self):
+        if self.h:
+            self.L.adapter_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RefAdapter:
+    """ctypes wrapper of oracle/_ref/libref_adapter.so."""
+
+    def __init__(self, path: str = ADAPTER_SO):
+        L = self.lib = C.CDLL(path)
+        L.adapter_scene.restype = C.c_void_p
+        L.adapter_scene.argtypes = [C.c_int]
+        L.adapter_scene_free.argtypes = [C.c_void_p]
+        L.adapter_supported.restype = C.c_int
+        L.adapter_supported.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.adapter_desc.restype = C.c_void_p
+        L.adapter_desc.argtypes = [C.c_void_p]
+        L.adapter_n_tris.restype = C.c_int64
+        L.adapter_n_tris.argtypes = [C.c_void_p]
+        L.adapter_camera_desc.argtypes = [C.c_int, C.POINTER(abi.CameraDesc)]
+        L.adapter_render_gpu.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_ulonglong, C.c_int, C.c_int,
+                                         C.c_int, _dp, _dp, _dp]
+        L.adapter_render_reference.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_ulonglong, C.c_int,
+                                               C.c_int, C.c_int, _dp, _dp, _dp]
+
+    def scene(self, which) -> AdapterScene:
+        return AdapterScene(self.lib, which)
+
+    def camera(self, res):
+        c = abi.CameraDesc()
+        self.lib.adapter_camera_desc(res, C.byref(c))
+        return c
+
+    def _render(self, fn, s, res, n, seed_base, hero_c, depth, extra):
+        xyz = np.zeros((res * res, 3))
+        hits = np.zeros(res * res)
+        e = np.zeros(5)
+        fn(s.h, res, n, seed_base, hero_c, depth, extra, _ptr(xyz, _dp), _ptr(hits, _dp), _ptr(e, _dp))
+        keys = ("emitted", "absorbed", "sensor", "escaped", "residual")
+        return xyz, hits, dict(zip(keys, e))
+
+    def render_gpu(self, s, res, n, seed_base=0, hero_c=4, depth=8, bvh=2):
+        """gpref::renderForward (the HIP library); bvh: GP_BVH_PREBUILT (2, the scene's
+        own tree) or GP_BVH_HOST_SAH (0)."""
+        return self._render(self.lib.adapter_render_gpu, s, res, n, seed_base, hero_c, depth, bvh)
+
+    def render_reference(self, s, res, n, seed_base=0, hero_c=4, depth=8, threads=1):
+        return self._render(self.lib.adapter_render_reference, s, res, n, seed_base, hero_c, depth, threads)

@@ -128,9 +128,23 @@ inline double planck(double kelvin, double lambdaNm) {
     double e = std::exp((h * c) / (l * kb * kelvin)) - 1.0;
     return (2.0 * h * c * c) / (std::pow(l, 5.0) * e);
 }
-double specEval(const gp_spectrum& s, double w) {
+// GP_SPEC_SAMPLED: linear interpolation of n uniform samples over [lo, hi], clamped to
+// the end samples outside it (the reference GPU backend's bakeSpec / specLookup,
+// render_cuda.cu:12658-12663, :1641-1649, at the caller's sample count and range).
+inline double sampledEval(const double* tab, int n, double lo, double hi, double w) {
+    double f = (w - lo) * (double)(n - 1) / (hi - lo);   // exact at the sample wavelengths
+    if (!(f > 0.0)) return tab[0];
+    if (f >= (double)(n - 1)) return tab[n - 1];
+    int i = (int)f;
+    double fr = f - (double)i;
+    return tab[i] * (1.0 - fr) + tab[i + 1] * fr;
+}
+// `pool` holds the samples of GP_SPEC_SAMPLED spectra (gp_scene_desc.spectrum_samples).
+double specEval(const gp_spectrum& s, double w, const double* pool = nullptr) {
     if (s.band_hi > s.band_lo && !(w >= s.band_lo && w <= s.band_hi)) return 0.0;
     switch (s.kind) {
+        case GP_SPEC_SAMPLED:
+            return pool ? sampledEval(pool + s.table, (int)s.p[2], s.p[0], s.p[1], w) : 0.0;
         case GP_SPEC_CONST: return s.p[0];
         case GP_SPEC_STEP: return (w < s.p[2]) ? s.p[0] : s.p[1];
         case GP_SPEC_GAUSS: {
@@ -177,14 +191,14 @@ double ybarIntegral() {
 struct LambdaCdf {
     std::vector<double> cdf;
     double integral = 0.0;
-    void build(const gp_spectrum& spd) {
+    void build(const gp_spectrum& spd, const double* pool = nullptr) {
         const double step = 1.0;
         int n = static_cast<int>((kLamMax - kLamMin) / step);
         cdf.assign(n + 1, 0.0);
         double acc = 0.0;
         for (int i = 0; i < n; ++i) {
             double w = kLamMin + (i + 0.5) * step;
-            acc += std::max(0.0, specEval(spd, w)) * step;
+            acc += std::max(0.0, specEval(spd, w, pool)) * step;
             cdf[i + 1] = acc;
         }
         integral = acc;
@@ -531,6 +545,7 @@ inline bool planarMirror(const Mat& m) {
 struct Scene {
     std::vector<Tri> tris;
     std::vector<Mat> mats;
+    std::vector<double> pool;   // samples of GP_SPEC_SAMPLED spectra
     std::vector<Light> lights;
     std::vector<double> lightCdf;
     std::vector<MirrorPlane> mirrors;
@@ -811,7 +826,7 @@ struct Tracer {
         ++k.splats;
         const double invD2Omega = 1.0 / (D * D * omega);
         for (int i = 0; i < nUp; ++i) {
-            const double refl = clampUnit(specEval(mm.reflect, lam[i]));
+            const double refl = clampUnit(specEval(mm.reflect, lam[i], S.pool.data()));
             const double contrib = beta[i] * (w[i] * shape) * refl * invD2Omega;
             if (contrib <= 0.0) continue;
             film.add(px, py, V3{xbar(lam[i]), ybar(lam[i]), zbar(lam[i])} * contrib);
@@ -825,7 +840,7 @@ struct Tracer {
     // Dielectric interface: Fresnel-weighted reflect or Snell refract (exterior
     // index 1), optionally frosted by a glossy lobe.
     void dielectric(const Mat& m, const Hit& h, V3& ro, V3& rd, double lambda, Pcg& rng) {
-        double ng = specEval(m.ior, lambda);
+        double ng = specEval(m.ior, lambda, S.pool.data());
         bool entering = dot(rd, h.ng) < 0.0;
         V3 nl = entering ? h.ng : neg(h.ng);
         double n1 = entering ? 1.0 : ng;
@@ -900,7 +915,7 @@ struct Tracer {
             }
             if (m.type == GP_MAT_MIRROR || m.type == GP_MAT_GLOSSY) {
                 double c[8] = {};
-                for (int i = 0; i < nUp; ++i) c[i] = clampUnit(specEval(m.reflect, lam[i]));
+                for (int i = 0; i < nUp; ++i) c[i] = clampUnit(specEval(m.reflect, lam[i], S.pool.data()));
                 const double q = maxLive(c, nUp);
                 if (roulette) {
                     if (rng.uniform() >= q) { e.absorbed += activeSum(); return; }
@@ -930,7 +945,7 @@ struct Tracer {
             }
             // Diffuse
             double rho[8];
-            for (int i = 0; i < nUp; ++i) rho[i] = clampUnit(specEval(m.reflect, lam[i]));
+            for (int i = 0; i < nUp; ++i) rho[i] = clampUnit(specEval(m.reflect, lam[i], S.pool.data()));
             V3 ngo = facingGeoN(h);
             V3 wi = neg(rd);
             splatHero(h.p, h.n, ngo, wi, lam, beta, rho, nUp);
@@ -962,6 +977,8 @@ struct Tracer {
 
 Scene* makeScene(const gp_scene_desc* d) {
     auto* s = new Scene();
+    if (d->spectrum_samples && d->n_spectrum_samples > 0)
+        s->pool.assign(d->spectrum_samples, d->spectrum_samples + d->n_spectrum_samples);
     for (int i = 0; i < d->n_materials; ++i) {
         Mat m;
         m.type = d->materials[i].type;
@@ -984,7 +1001,7 @@ Scene* makeScene(const gp_scene_desc* d) {
         Light L;
         L.origin = v3(ge.origin); L.u = v3(ge.u); L.v = v3(ge.v); L.normal = v3(ge.normal);
         L.area = ge.area; L.collimated = ge.collimated != 0; L.beam = v3(ge.beam_dir);
-        L.cdf.build(ge.spd);
+        L.cdf.build(ge.spd, s->pool.data());
         L.integral = L.cdf.integral;
         s->lights.push_back(std::move(L));
     }
@@ -1212,23 +1229,23 @@ void or_seed_unit_uniforms(uint64_t k, uint64_t salt, int n, double* out) {
     for (int i = 0; i < n; ++i) out[i] = r.uniform();
 }
 uint64_t or_mix64(uint64_t x) { return splitmix(x); }
-void or_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out) {
-    for (int i = 0; i < n; ++i) out[i] = specEval(*s, lam[i]);
+void or_spectrum_eval(const gp_spectrum* s, const double* pool, const double* lam, int n, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = specEval(*s, lam[i], pool);
 }
 void or_cie(const double* lam, int n, double* xyz) {
     for (int i = 0; i < n; ++i) { xyz[3 * i] = xbar(lam[i]); xyz[3 * i + 1] = ybar(lam[i]); xyz[3 * i + 2] = zbar(lam[i]); }
 }
 double or_cie_y_integral() { return ybarIntegral(); }
-void or_emission_sample(const gp_spectrum* s, const double* u, int n, double* lam, double* pdf,
-                        double* integral) {
+void or_emission_sample(const gp_spectrum* s, const double* pool, const double* u, int n, double* lam,
+                        double* pdf, double* integral) {
     LambdaCdf c;
-    c.build(*s);
+    c.build(*s, pool);
     for (int i = 0; i < n; ++i) { pdf[i] = 0.0; lam[i] = c.invert(u[i], pdf[i]); }
     if (integral) *integral = c.integral;
 }
-int or_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double* pdf) {
+int or_sample_bundle(const gp_spectrum* s, const double* pool, double u, int C, double* lam, double* pdf) {
     LambdaCdf c;
-    c.build(*s);
+    c.build(*s, pool);
     return drawBundle(c, u, C, lam, pdf) ? 1 : 0;
 }
 void or_cosine_hemisphere(const double* nrm, uint64_t seq, uint64_t seed, int n, double* out) {

@@ -54,7 +54,8 @@ int or_occluded(void* s, const double* rays, const double* maxDist, int64_t n, u
 void or_pcg_stream(uint64_t seq, uint64_t seed, int n, uint32_t* out);
 void or_seed_unit_uniforms(uint64_t k, uint64_t salt, int n, double* out);
 uint64_t or_mix64(uint64_t x);
-void or_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out);
+// pool: samples of GP_SPEC_SAMPLED spectra (may be null for the analytic kinds)
+void or_spectrum_eval(const gp_spectrum* s, const double* pool, const double* lam, int n, double* out);
 void or_cie(const double* lam, int n, double* xyz);
 double or_cie_y_integral();
 uint64_t or_checkpoint_guard(int64_t nTris, int64_t nSpheres, int64_t nEmitters, double totalPower,
@@ -64,9 +65,9 @@ int or_read_checkpoint(const char* path, int res, int resY, uint64_t guard, int 
 void or_film_to_linear(const double* xyz, int64_t npix, double N, double* lin);
 void or_film_to_rgb8(const double* xyz, int W, int H, double N, double expComp, int absolute,
                      double* lockAnchor, uint8_t* img, double* eAutoOut, double* exposureOut);
-void or_emission_sample(const gp_spectrum* s, const double* u, int n, double* lam, double* pdf,
+void or_emission_sample(const gp_spectrum* s, const double* pool, const double* u, int n, double* lam, double* pdf,
                         double* integral);
-int or_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double* pdf);
+int or_sample_bundle(const gp_spectrum* s, const double* pool, double u, int C, double* lam, double* pdf);
 void or_cosine_hemisphere(const double* nrm, uint64_t seq, uint64_t seed, int n, double* out);
 void or_sample_glossy(const double* mdir, double rough, uint64_t seq, uint64_t seed, int n,
                       double* out);

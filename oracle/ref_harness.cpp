@@ -29,9 +29,28 @@
 
 namespace {
 
-Spectrum toSpectrum(const gp_spectrum& s) {
+// `pool`: the samples of GP_SPEC_SAMPLED spectra (gp_scene_desc.spectrum_samples);
+// a sampled spectrum becomes a closure interpolating its own copy of them, as the
+// reference's GPU backend interpolates its baked tables (specLookup,
+// src/render_cuda.cu:1641-1649).
+Spectrum toSpectrum(const gp_spectrum& s, const double* pool = nullptr) {
     Spectrum base;
     switch (s.kind) {
+        case GP_SPEC_SAMPLED: {
+            const int n = (int)s.p[2];
+            const double lo = s.p[0], hi = s.p[1];
+            std::vector<double> tab(pool ? pool + s.table : nullptr, pool ? pool + s.table + n : nullptr);
+            if (tab.empty()) { base = constantSpectrum(0.0); break; }
+            base = [tab, n, lo, hi](double w) {
+                double f = (w - lo) * (double)(n - 1) / (hi - lo);
+                if (!(f > 0.0)) return tab[0];
+                if (f >= (double)(n - 1)) return tab[(size_t)n - 1];
+                int i = (int)f;
+                double fr = f - (double)i;
+                return tab[(size_t)i] * (1.0 - fr) + tab[(size_t)i + 1] * fr;
+            };
+            break;
+        }
         case GP_SPEC_CONST: base = constantSpectrum(s.p[0]); break;
         case GP_SPEC_STEP: {
             const double lo = s.p[0], hi = s.p[1], edge = s.p[2];
@@ -86,8 +105,8 @@ void* ref_scene_create(const gp_scene_desc* d) {
         const gp_material& gm = d->materials[i];
         Material m;
         m.type = static_cast<MatType>(gm.type);
-        m.reflect = toSpectrum(gm.reflect);
-        m.ior = toSpectrum(gm.ior);
+        m.reflect = toSpectrum(gm.reflect, d->spectrum_samples);
+        m.ior = toSpectrum(gm.ior, d->spectrum_samples);
         m.roughness = gm.roughness;
         s.mats.push_back(m);
     }
@@ -102,7 +121,7 @@ void* ref_scene_create(const gp_scene_desc* d) {
         const gp_emitter& e = d->emitters[i];
         s.addAreaLight(Vec3{e.origin[0], e.origin[1], e.origin[2]},
                        Vec3{e.u[0], e.u[1], e.u[2]}, Vec3{e.v[0], e.v[1], e.v[2]},
-                       Vec3{e.normal[0], e.normal[1], e.normal[2]}, e.area, toSpectrum(e.spd),
+                       Vec3{e.normal[0], e.normal[1], e.normal[2]}, e.area, toSpectrum(e.spd, d->spectrum_samples),
                        1.0, e.collimated != 0,
                        Vec3{e.beam_dir[0], e.beam_dir[1], e.beam_dir[2]});
     }
@@ -237,8 +256,8 @@ void ref_seed_unit_uniforms(uint64_t k, uint64_t salt, int n, double* out) {
 }
 uint64_t ref_mix64(uint64_t x) { return mix64(x); }
 
-void ref_spectrum_eval(const gp_spectrum* s, const double* lam, int n, double* out) {
-    Spectrum f = toSpectrum(*s);
+void ref_spectrum_eval(const gp_spectrum* s, const double* pool, const double* lam, int n, double* out) {
+    Spectrum f = toSpectrum(*s, pool);
     for (int i = 0; i < n; ++i) out[i] = f(lam[i]);
 }
 void ref_cie(const double* lam, int n, double* xyz) {
@@ -246,14 +265,14 @@ void ref_cie(const double* lam, int n, double* xyz) {
 }
 double ref_cie_y_integral() { return cieYIntegral(); }
 
-void ref_emission_sample(const gp_spectrum* s, const double* u, int n, double* lam, double* pdf,
-                         double* integral) {
-    EmissionSampler es; es.build(toSpectrum(*s), 1.0);
+void ref_emission_sample(const gp_spectrum* s, const double* pool, const double* u, int n, double* lam,
+                         double* pdf, double* integral) {
+    EmissionSampler es; es.build(toSpectrum(*s, pool), 1.0);
     for (int i = 0; i < n; ++i) { pdf[i] = 0.0; lam[i] = es.sampleAt(u[i], pdf[i]); }
     if (integral) *integral = es.integral;
 }
-int ref_sample_bundle(const gp_spectrum* s, double u, int C, double* lam, double* pdf) {
-    EmissionSampler es; es.build(toSpectrum(*s), 1.0);
+int ref_sample_bundle(const gp_spectrum* s, const double* pool, double u, int C, double* lam, double* pdf) {
+    EmissionSampler es; es.build(toSpectrum(*s, pool), 1.0);
     return hero::sampleBundle(es, u, C, lam, pdf) ? 1 : 0;
 }
 // n cosine-weighted directions about `nrm` (src/rng.h:45-52), each drawn from a

@@ -86,8 +86,9 @@ def classify_prim_mismatches(orc, cs, rays, gprim, gt, cprim, ct, tmin=1e-6):
         g, c = gprim[diff], cprim[diff]
         t, tq, trob = a["t"], a["tq"], a["t_robust"]
         gtd = np.asarray(gt, np.float64)[diff]
-        t_ok = np.isfinite(tq) & (np.abs(gtd - tq) <= 2e-6 * (1.0 + np.where(np.isfinite(tq), tq, 0.0)))
-        near = (c >= 0) & (g >= 0) & t_ok & (np.abs(tq - t) <= TIE_REL * np.abs(t))
+        with np.errstate(invalid="ignore"):   # inf - inf on misses: compares False
+            t_ok = np.isfinite(tq) & (np.abs(gtd - tq) <= 2e-6 * (1.0 + np.where(np.isfinite(tq), tq, 0.0)))
+            near = (c >= 0) & (g >= 0) & t_ok & (np.abs(tq - t) <= TIE_REL * np.abs(t))
         behind = a["edge"] & (g >= 0) & t_ok & (tq >= t * (1 - TIE_REL)) & (tq <= trob * (1 + TIE_REL) + 2e-6)
         front = (g >= 0) & a["q_graze"] & t_ok & (tq <= t * (1 + TIE_REL))
         gpu_miss = a["edge"] & (g < 0) & ~np.isfinite(trob)
@@ -130,3 +131,41 @@ def host_threads():
     if share.isdigit() and int(share) > 0:
         n = min(n, int(share))
     return max(1, n)
+
+
+SAMPLED_LO, SAMPLED_HI, SAMPLED_N = 360.5, 829.5, 470
+
+
+def to_sampled(sd, orc):
+    """The same scene with every material and emitter spectrum replaced by its values at
+    the 470 emission-bin centres 360.5 .. 829.5 nm (SPEC_SAMPLED, values from the
+    oracle's analytic evaluation), the form in which the reference's Spectrum
+    functions reach the core through the adapter."""
+    from goodphotons_amd import abi
+    lam = SAMPLED_LO + np.arange(SAMPLED_N, dtype=np.float64)
+    pool = []
+
+    def conv(spec):
+        s = abi.Spectrum()
+        s.kind = abi.SPEC_SAMPLED
+        s.table = SAMPLED_N * len(pool)
+        s.p[0], s.p[1], s.p[2] = SAMPLED_LO, SAMPLED_HI, SAMPLED_N
+        pool.append(orc.spectrum_eval(spec, lam))
+        return s
+
+    mats = []
+    for m in sd.materials:
+        m2 = type(m).from_buffer_copy(m)
+        m2.reflect = conv(m.reflect)
+        m2.ior = conv(m.ior)
+        mats.append(m2)
+    ems = []
+    for e in sd.emitters:
+        e2 = type(e).from_buffer_copy(e)
+        e2.spd = conv(e.spd)
+        ems.append(e2)
+    out = sd.with_camera_res(sd.camera.res_x, sd.camera.res_y)
+    out.name = sd.name + "_sampled"
+    out.materials, out.emitters = mats, ems
+    out.spectrum_samples = np.concatenate(pool)
+    return out

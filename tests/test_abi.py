@@ -33,10 +33,11 @@ def test_library_exports_every_symbol():
         assert getattr(lib, name) is not None
 
 
-STRUCTS = ["gp_spectrum", "gp_material", "gp_emitter", "gp_scene_desc", "gp_camera_desc",
+STRUCTS = ["gp_spectrum", "gp_material", "gp_emitter", "gp_bvh_node", "gp_scene_desc", "gp_camera_desc",
            "gp_energy", "gp_trace_params", "gp_trace_stats", "gp_device_info", "gp_scene_info",
            "gp_tuning", "gp_server_config", "gp_job_desc", "gp_job_result"]
 CTYPES = {"gp_spectrum": abi.Spectrum, "gp_material": abi.Material, "gp_emitter": abi.Emitter,
+          "gp_bvh_node": abi.BvhNode,
           "gp_scene_desc": abi.SceneDesc, "gp_camera_desc": abi.CameraDesc, "gp_energy": abi.Energy,
           "gp_trace_params": abi.TraceParams, "gp_trace_stats": abi.TraceStats,
           "gp_device_info": abi.DeviceInfo, "gp_scene_info": abi.SceneInfo, "gp_tuning": abi.Tuning,

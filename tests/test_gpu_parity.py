@@ -14,7 +14,7 @@ import pytest
 
 import goodphotons_amd as gp
 from goodphotons_amd import abi, scenes
-from tests.parity import check_prim_parity, image_metrics, seeded_rays
+from tests.parity import check_prim_parity, image_metrics, seeded_rays, to_sampled
 
 pytestmark = pytest.mark.gpu
 
@@ -113,6 +113,31 @@ def test_film_matches_oracle(gplib, orc, name, builder, n, depth, rr):
     assert abs(gh.sum() / ch.sum() - 1) < 2e-3
     for k in ("absorbed", "escaped", "residual"):
         assert abs(ge[k] - ce[k]) <= 2e-3 * ce["emitted"], k
+
+
+@pytest.mark.parametrize("name,make,n,depth", [("c0", lambda: scenes.cornell_c0(64), 1 << 18, 8),
+                                               ("c1", lambda: scenes.caustics_c1(64), 1 << 18, 16)],
+                         ids=["c0", "c1"])
+def test_sampled_spectra_film_matches_oracle(gplib, orc, name, make, n, depth):
+    """Every spectrum as 470 samples (SPEC_SAMPLED, the form the reference's Spectrum
+    functions take through the adapter): the device interpolates float samples, the
+    oracle double ones; same photons, same bar as the analytic spectra. The sampled
+    scene also stays close to the analytic one (interpolation between 1-nm samples)."""
+    sd = make()
+    ss = to_sampled(sd, orc)
+    gs = gp.GpuScene(ss, gplib)
+    gx, gh, ge, gst = gp.render_forward(gs, ss.camera, n, hero_c=4, max_depth=depth, rr_depth=sd.rr_depth)
+    cx, ch, ce, cst = orc.render(orc.scene(ss), ss.camera, n, hero_c=4, max_bounce=depth, rr_depth=sd.rr_depth,
+                                 threads=8)
+    ax, ah, ae, ast = orc.render(orc.scene(sd), sd.camera, n, hero_c=4, max_bounce=depth, rr_depth=sd.rr_depth,
+                                 threads=8)
+    m = image_metrics(gx, cx, n, n, res=(64, 64))
+    a = image_metrics(cx, ax, n, n, res=(64, 64))
+    print(f"[film] {name} sampled: gpu vs oracle {m}; sampled vs analytic (oracle) {a}", flush=True)
+    assert m["rel_rmse_max"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
+    assert abs(ge["emitted"] / ce["emitted"] - 1) < 1e-6
+    assert abs(gst["segments"] / cst["segments"] - 1) < 2e-3
+    assert a["rel_rmse_max"] < 0.01 and abs(a["mean_lum_ratio"] - 1) < 0.003
 
 
 def test_energy_ledger_closes(gplib):
