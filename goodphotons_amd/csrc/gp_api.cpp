@@ -666,6 +666,8 @@ gp_status gp_set_tuning(const gp_tuning* in) {
         return fail(GP_ERR_INVALID, "iters_per_graph must be 2..64 (1..64 without graphs)");
     if (in->lds_stack != 8 && in->lds_stack != 16 && in->lds_stack != 24 && in->lds_stack != 32)
         return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
+    if ((in->block_extend != 0 && in->block_extend != 256) || (in->block_shade != 0 && in->block_shade != 256))
+        return fail(GP_ERR_INVALID, "block_extend and block_shade are fixed at 256 (0 or 256)");
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
     if (E.owner) return fail(GP_ERR_STATE, "a server has jobs in flight on this device");

@@ -9,10 +9,11 @@
   python -m goodphotons_amd.workloads c4 [--jobs J]
       Poisson arrivals (20 jobs/s, seeded) over 8 resident scenes (c0, c1, c2, a c3
       frame, 4 random soups of 1e5..2e6 tris), films from {128,256,512,1024}^2,
-      log-uniform budgets 2^20..2^26: continuous-batching server vs one job at a
-      time, both with the reference's plain-wavefront scheduling (2^20 slots, host
-      read per pass, no graph) and with this engine's tuned gp_trace; under the
-      arrival trace and saturated; p50/p99 latency and aggregate segments/s.
+      log-uniform budgets 2^20..2^26: continuous-batching server (shortest remaining
+      job first, and in admission order) vs one job at a time, both with the
+      reference's plain-wavefront scheduling (2^20 slots, host read per pass, no graph)
+      and with this engine's tuned gp_trace; under the arrival trace and saturated;
+      p50/p99 latency and aggregate segments/s.
 Each prints one JSON line.
 """
 from __future__ import annotations
@@ -184,10 +185,10 @@ def _run_plain(lib, sc, jobs, arrivals, tuning=None):
             lib.set_tuning(**saved)
 
 
-def _run_server(lib, sc, jobs, arrivals):
+def _run_server(lib, sc, jobs, arrivals, order_policy=gp.abi.ORDER_SHORTEST_FIRST):
     """Continuous batching: submit on arrival, keep the device fed, take each result
     (film downloaded) as soon as it finishes."""
-    srv = gp.Server(hbm_budget_bytes=8 << 30, max_live_jobs=64, lib=lib)
+    srv = gp.Server(hbm_budget_bytes=8 << 30, max_live_jobs=64, lib=lib, order_policy=order_policy)
     try:
         t0 = time.perf_counter()
         live = {}
@@ -230,6 +231,7 @@ def run_c4(n_jobs=60, lam=20.0, full=True):
            "reference_style_tuning": REFERENCE_STYLE_TUNING}
     for mode, arrivals in (("poisson", [j["arrival"] for j in jobs]), ("saturated", [0.0] * len(jobs))):
         r = {"server": _run_server(lib, sc, jobs, arrivals),
+             "server_admission_order": _run_server(lib, sc, jobs, arrivals, gp.abi.ORDER_ADMISSION),
              "plain_reference_style": _run_plain(lib, sc, jobs, arrivals, REFERENCE_STYLE_TUNING),
              "plain_tuned": _run_plain(lib, sc, jobs, arrivals)}
         r["throughput_ratio_vs_reference_style"] = (r["server"]["segments_per_s"] /

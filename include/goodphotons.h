@@ -377,12 +377,15 @@ gp_status gp_probe_any_hit(gp_scene* scene, const float* rays, const float* tmax
 /* ---- wavefront tuning --------------------------------------------------- */
 typedef struct gp_tuning {
     int32_t queue_capacity;  /* paths in flight (power of two, default 2^23)         */
-    int32_t block_extend;    /* threads per block of the traversal kernels           */
-    int32_t block_shade;     /* threads per block of emit/shade                      */
+    int32_t block_extend;    /* threads per block of the traversal kernels: fixed at
+                                256 (the LDS stacks, splat tables and per-job tables
+                                are laid out for 256-lane workgroups); 0 or 256     */
+    int32_t block_shade;     /* threads per block of emit/shade: fixed at 256 (0/256) */
     int32_t grid_waves_per_cu; /* waves per CU the fixed grids launch (workgroups =
-                                  CUs x this / 4); beyond what fits at once they run
-                                  in later rounds, which evens out the grid-stride
-                                  loops (default 64)                                  */
+                                  CUs x this / 4), 4..64; beyond what fits at once they
+                                  run in later rounds, which evens out the grid-stride
+                                  loops (default 64); 16 = the resident wave slots of
+                                  the traversal kernels (persistent threads)         */
     int32_t iters_per_graph; /* iterations per host read of the wavefront state (per
                                 captured hipGraph: even, 2..64; 1..64 without graphs) */
     int32_t use_graph;       /* 1 = replay captured graphs, 0 = plain launches       */
