@@ -72,7 +72,7 @@ class TraceParams(C.Structure):
 class TraceStats(C.Structure):
     _fields_ = [("photons", C.c_uint64), ("segments", C.c_uint64), ("connections", C.c_uint64),
                 ("splats", C.c_uint64), ("iterations", C.c_uint64), ("wall_ms", C.c_double),
-                ("kernel_ms", C.c_double * 5), ("kernel_launches", C.c_uint64 * 5),
+                ("kernel_ms", C.c_double * 6), ("kernel_launches", C.c_uint64 * 6),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("direct_connections", C.c_uint64)]
@@ -100,7 +100,8 @@ class Tuning(C.Structure):
     _fields_ = [("queue_capacity", C.c_int32), ("block_extend", C.c_int32),
                 ("block_shade", C.c_int32), ("grid_waves_per_cu", C.c_int32),
                 ("iters_per_graph", C.c_int32), ("use_graph", C.c_int32),
-                ("lds_stack", C.c_int32), ("profile", C.c_int32)]
+                ("lds_stack", C.c_int32), ("profile", C.c_int32),
+                ("ray_sort", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -158,4 +159,4 @@ def vec3(arr, vals):
     for i in range(3):
         arr[i] = float(vals[i])
 
-KERNEL_NAMES = ["k_schedule", "k_emit", "k_extend", "k_shade", "k_connect"]
+KERNEL_NAMES = ["k_schedule", "k_emit", "k_extend", "k_shade", "k_connect", "k_bin"]

@@ -238,6 +238,23 @@ static gp_status createScene(const gp_scene_desc* d, int builder, gp_scene** out
         trin[k] = make_float4((float)c[0], (float)c[1], (float)c[2], 0.f);
         std::memcpy(&trin[k].w, &mat, 4);
     };
+    {
+        // ray-binning grid: kBinCellsPerAxis cells per axis over the centroid box
+        float blo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (size_t k = 0; k < n; ++k) {
+            const float* v = d->verts + 9 * k;
+            for (int a = 0; a < 3; ++a) {
+                const float c = 0.5f * (std::min({v[a], v[3 + a], v[6 + a]}) + std::max({v[a], v[3 + a], v[6 + a]}));
+                blo[a] = std::min(blo[a], c);
+                bhi[a] = std::max(bhi[a], c);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            const float ext = bhi[a] - blo[a];
+            s->binLo[a] = blo[a];
+            s->binScale[a] = (ext > 0.f && std::isfinite(ext)) ? (float)kBinCellsPerAxis / ext : 0.f;
+        }
+    }
     DeviceBvh dbvh;
     std::vector<float> quads;   // four-wide nodes of the host-built tree
     if (builder == GP_BVH_DEVICE_LBVH) {
@@ -560,7 +577,7 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     }
     auto t1 = std::chrono::steady_clock::now();
     acc.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kKernelSlots; ++k) {
         acc.kernel_ms[k] = E.kernelMs[k];
         acc.kernel_launches[k] = E.kernelLaunches[k];
     }
@@ -668,6 +685,7 @@ gp_status gp_set_tuning(const gp_tuning* in) {
         return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
     if ((in->block_extend != 0 && in->block_extend != 256) || (in->block_shade != 0 && in->block_shade != 256))
         return fail(GP_ERR_INVALID, "block_extend and block_shade are fixed at 256 (0 or 256)");
+    if (in->ray_sort != 0 && in->ray_sort != 1) return fail(GP_ERR_INVALID, "ray_sort must be 0 or 1");
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
     if (E.owner) return fail(GP_ERR_STATE, "a server has jobs in flight on this device");

@@ -3,6 +3,8 @@
 // One iteration of the wavefront:
 //   k_schedule  (1 block)  refill free queue slots from the jobs' photon budgets
 //   k_emit                 emission + hero bundle + direct light->camera connection
+//   k_bin_*                (gp_tuning.ray_sort only) counting sort of the paths by
+//                          origin cell and direction octant into a third queue
 //   k_extend               closest hit for every live path (four-wide nodes, LDS stack)
 //   k_shade                material scatter, Russian roulette, camera connection,
 //                          survivors compacted into the other queue by wave ballot
@@ -133,6 +135,16 @@ __device__ __forceinline__ unsigned segAppend(unsigned (*cnt)[kLine], unsigned c
         return ~0u;
     }
     return g * cap + off;
+}
+
+// Workgroups are dispatched to the eight XCDs round-robin (workgroup b to XCD b % 8),
+// and each XCD has its own L2. Renumbering b as (b % 8) * (G / 8) + b / 8 gives every
+// XCD a contiguous eighth of each grid-stride round (and adjacent numbers to the
+// workgroups of one XCD). Identity when the grid G is not a multiple of 8.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcdBlock() {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    return (G % kXcds) ? b : (b % kXcds) * (G / kXcds) + b / kXcds;
 }
 
 // Camera connection record for a diffuse (or emitter) vertex at p with shading
@@ -401,6 +413,7 @@ __global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rb = q.binned ? 2 : cur;
     Stack<SD, true, SPILL> stk{sStk + threadIdx.x, nullptr,
                                SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
@@ -409,15 +422,15 @@ __global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
     JobAcc<NA> sacc;
     AccAddr sa{q.jobs, kAccSegments};
     for (unsigned li = gtid; li < n; li += stride) {
-        const unsigned i = sm.slot(li);
-        const uint4 s4 = q.st[cur][i];
+        const unsigned i = q.binned ? li : sm.slot(li);
+        const uint4 s4 = q.st[rb][i];
         int best = -1;
         float t = INFINITY;
         if (!(s4.w & 0x80000000u)) {
             const int j = (int)(s4.w & 0xffffu);
             const DevJob& J = q.jobs[j];
-            const float4 o = q.ro[cur][i];
-            const float4 d = q.rd[cur][i];
+            const float4 o = q.ro[rb][i];
+            const float4 d = q.rd[rb][i];
             RayPre r = prepareRay(xyz(o), xyz(d));
             float tmax = INFINITY;
             TravCount<STATS> tc;
@@ -448,9 +461,10 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     JobAcc<3> eacc;   // absorbed, escaped, residual -> ledger entries 1, 3, 4
     AccAddr ea{q.jobs, kAccEnergy};
     auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
+    const int rb = q.binned ? 2 : cur;
     for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
-        const unsigned i = sm.slot(li);
-        const uint4 s4 = q.st[cur][i];
+        const unsigned i = q.binned ? li : sm.slot(li);
+        const uint4 s4 = q.st[rb][i];
         bool alive = false;
         // diffuse vertex to connect to the cameras after shading
         int camN = 0;
@@ -471,7 +485,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             int nUp = (int)((meta >> 24) & 7u);
             const DevJob& J = q.jobs[j];
             accumulate(eacc, j, eTab);
-            const float4 l4 = q.lam[cur][i], b4 = q.beta[cur][i];
+            const float4 l4 = q.lam[rb][i], b4 = q.beta[rb][i];
             lam[0] = l4.x; lam[1] = l4.y; lam[2] = l4.z; lam[3] = l4.w;
             beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z; beta[3] = b4.w;
             auto activeSum = [&]() {
@@ -484,7 +498,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             if (tri < 0) {
                 eacc.v[1] += toFixed(activeSum(), J.energyScale);
             } else {
-                const float4 o4 = q.ro[cur][i], d4 = q.rd[cur][i];
+                const float4 o4 = q.ro[rb][i], d4 = q.rd[rb][i];
                 const F3 d = xyz(d4);
                 const F3 p = xyz(o4) + d * h.x;
                 const float4 tn = J.trin[tri];
@@ -922,6 +936,173 @@ __global__ void __launch_bounds__(kTravThreads) k_mirror(Queues q) {
     flushJobTable(cacc, cTab, nJobs, ca);
 }
 
+// ------------------------------------------------------------------ ray binning
+// Counting sort of queue `cur` into queue 2 by a 9-bit key: the Morton code of the
+// origin's cell in a 4 x 4 x 4 grid over the job's triangle-centroid box, then the
+// direction octant. Column c (workgroup xcdBlock() of C = q.binCols) owns the
+// contiguous logical range [n c / C, n (c + 1) / C). The order inside a (column,
+// bin) run is that of LDS atomics; nothing downstream depends on it (films, ledgers
+// and counters are integer sums).
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+
+__device__ __forceinline__ unsigned binCell(float x, float lo, float scale) {
+    // fmaxf returns 0 for a NaN cell coordinate
+    const float f = fminf(fmaxf((x - lo) * scale, 0.0f), (float)(kBinCellsPerAxis - 1));
+    return (unsigned)f;
+}
+
+__device__ __forceinline__ unsigned binKey(const Queues& q, uint4 s4, float4 o, float4 d) {
+    if (s4.w & 0x80000000u) return 0u;
+    const DevJob& J = q.jobs[s4.w & 0xffffu];
+    const unsigned m = spread3(binCell(o.x, J.binLo[0], J.binScale[0])) |
+                       (spread3(binCell(o.y, J.binLo[1], J.binScale[1])) << 1) |
+                       (spread3(binCell(o.z, J.binLo[2], J.binScale[2])) << 2);
+    const unsigned oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return (m << 3) | oct;
+}
+
+__device__ __forceinline__ void binRange(unsigned n, unsigned col, unsigned cols, unsigned& lo, unsigned& hi) {
+    lo = (unsigned)(((unsigned long long)n * col) / cols);
+    hi = (unsigned)(((unsigned long long)n * (col + 1)) / cols);
+}
+
+// Histogram of column xcdBlock() into binHist[col][*].
+__global__ void __launch_bounds__(256) k_bin_count(Queues q, int cur) {
+    __shared__ unsigned h[kBins];
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    SegMap sm;
+    const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
+    const unsigned col = xcdBlock();
+    unsigned lo, hi;
+    binRange(n, col, (unsigned)q.binCols, lo, hi);
+    for (unsigned li = lo + threadIdx.x; li < hi; li += blockDim.x) {
+        const unsigned i = sm.slot(li);
+        atomicAdd(&h[binKey(q, q.st[cur][i], q.ro[cur][i], q.rd[cur][i])], 1u);
+    }
+    __syncthreads();
+    unsigned* out = q.binHist + (size_t)col * kBins;
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) out[b] = h[b];
+}
+
+// Per bin: exclusive prefix of the columns inside each of the kBinChunks column
+// chunks (in place), the chunks' exclusive prefix (binChunk[chunk][bin]) and the
+// bin's total (binTotal[bin]). A workgroup takes 32 bins x kBinChunks chunks at a
+// time, grid-stride over the kBins / 32 groups.
+__global__ void __launch_bounds__(256) k_bin_chunks(Queues q) {
+    const unsigned per = (unsigned)q.binCols / kBinChunks;
+    __shared__ unsigned part[kBinChunks][32];
+    const unsigned bl = threadIdx.x & 31u, ch = threadIdx.x >> 5;
+    for (unsigned grp = blockIdx.x; grp < kBins / 32; grp += gridDim.x) {
+        const unsigned bin = grp * 32u + bl;
+        unsigned run = 0;
+        // eight columns loaded before any is rewritten, then the rest one at a time
+        unsigned c = ch * per;
+        const unsigned cEnd = (ch + 1) * per;
+        for (; c + 8 <= cEnd; c += 8) {
+            unsigned v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = q.binHist[(size_t)(c + k) * kBins + bin];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                q.binHist[(size_t)(c + k) * kBins + bin] = run;
+                run += v[k];
+            }
+        }
+        for (; c < cEnd; ++c) {
+            unsigned* e = q.binHist + (size_t)c * kBins + bin;
+            const unsigned v = *e;
+            *e = run;
+            run += v;
+        }
+        part[ch][bl] = run;
+        __syncthreads();
+        if (ch == 0) {
+            unsigned acc = 0;
+            for (unsigned k = 0; k < kBinChunks; ++k) {
+                q.binChunk[(size_t)k * kBins + bin] = acc;
+                acc += part[k][bl];
+            }
+            q.binTotal[bin] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of the kBins totals in place (one workgroup of 1024 threads).
+constexpr int kBinScanThreads = kBins < 1024 ? kBins : 1024;
+__global__ void __launch_bounds__(kBinScanThreads) k_bin_scan(Queues q) {
+    constexpr int kPer = kBins / kBinScanThreads;
+    __shared__ unsigned sum[kBinScanThreads];
+    const unsigned t = threadIdx.x;
+    unsigned v[kPer], acc = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        v[k] = q.binTotal[t * kPer + k];
+        acc += v[k];
+    }
+    sum[t] = acc;
+    __syncthreads();
+    for (unsigned off = 1; off < kBinScanThreads; off <<= 1) {
+        const unsigned add = t >= off ? sum[t - off] : 0u;
+        __syncthreads();
+        sum[t] += add;
+        __syncthreads();
+    }
+    unsigned base = t ? sum[t - 1] : 0u;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        q.binTotal[t * kPer + k] = base;
+        base += v[k];
+    }
+}
+
+// Moves column xcdBlock()'s paths to their bins' slots of queue 2.
+__global__ void __launch_bounds__(256) k_bin_scatter(Queues q, int cur) {
+    __shared__ unsigned slot[kBins];
+    const unsigned col = xcdBlock();
+    const unsigned ch = col / ((unsigned)q.binCols / kBinChunks);
+    const unsigned* colHist = q.binHist + (size_t)col * kBins;
+    const unsigned* chunk = q.binChunk + (size_t)ch * kBins;
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) slot[b] = q.binTotal[b] + chunk[b] + colHist[b];
+    __syncthreads();
+    SegMap sm;
+    const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
+    const unsigned room = kSegs * q.segCap;
+    unsigned lo, hi;
+    binRange(n, col, (unsigned)q.binCols, lo, hi);
+    const uint4* __restrict__ sst = q.st[cur];
+    const float4* __restrict__ sro = q.ro[cur];
+    const float4* __restrict__ srd = q.rd[cur];
+    const float4* __restrict__ slam = q.lam[cur];
+    const float4* __restrict__ sbeta = q.beta[cur];
+    uint4* __restrict__ dst_st = q.st[2];
+    float4* __restrict__ dro = q.ro[2];
+    float4* __restrict__ drd = q.rd[2];
+    float4* __restrict__ dlam = q.lam[2];
+    float4* __restrict__ dbeta = q.beta[2];
+    // two paths per step: both loaded before either is stored
+    for (unsigned li = lo + threadIdx.x; li < hi; li += 2 * blockDim.x) {
+        const unsigned lj = li + blockDim.x;
+        const bool two = lj < hi;
+        const unsigned i0 = sm.slot(li), i1 = two ? sm.slot(lj) : i0;
+        const uint4 s0 = sst[i0], s1 = sst[i1];
+        const float4 o0 = sro[i0], o1 = sro[i1], d0 = srd[i0], d1 = srd[i1];
+        const float4 l0 = slam[i0], l1 = slam[i1], b0 = sbeta[i0], b1 = sbeta[i1];
+        const unsigned k0 = binKey(q, s0, o0, d0), k1 = binKey(q, s1, o1, d1);
+        const unsigned t0 = atomicAdd(&slot[k0], 1u);
+        const unsigned t1 = two ? atomicAdd(&slot[k1], 1u) : 0u;
+        if (t0 >= room || (two && t1 >= room)) {   // impossible: the two passes see the same keys
+            q.ctrl->overflow = 1u;
+            continue;
+        }
+        dro[t0] = o0; drd[t0] = d0; dlam[t0] = l0; dbeta[t0] = b0; dst_st[t0] = s0;
+        if (two) { dro[t1] = o1; drd[t1] = d1; dlam[t1] = l1; dbeta[t1] = b1; dst_st[t1] = s1; }
+    }
+}
+
 // ------------------------------------------------------------------ probes
 template <int SD, bool SPILL>
 __global__ void __launch_bounds__(kTravThreads) k_probe_closest(const float4* nodes, const float4* tris,
@@ -1032,6 +1213,12 @@ void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s
     else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
 #undef GP_CALLS
 #undef GP_CALL
+}
+void launchBin(const Queues& q, int cur, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_count, dim3(q.binCols), dim3(256), 0, s, q, cur);
+    hipLaunchKernelGGL(k_bin_chunks, dim3(std::min(kBins / 32, q.binCols)), dim3(kBinChunks * 32), 0, s, q);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kBinScanThreads), 0, s, q);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(q.binCols), dim3(256), 0, s, q, cur);
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
 #define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP, false>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)

@@ -9,13 +9,16 @@ namespace gp {
 // Device pointers of one wavefront's queues. Path queues are double-buffered:
 // iteration k reads queue (k & 1) and writes survivors to the other. Each queue is
 // kSegs segments of segCap (paths) / connSegCap (connections) slots; a logical index
-// maps to (segment, offset) through the prefix of the per-segment counts.
+// maps to (segment, offset) through the prefix of the per-segment counts. With ray
+// binning (binned = 1) a third path queue holds the iteration's paths in bin order,
+// packed from slot 0 (logical index = slot): the closest-hit and shading kernels read
+// it instead of queue (k & 1).
 struct Queues {
-    float4* ro[2];
-    float4* rd[2];
-    float4* lam[2];
-    float4* beta[2];
-    uint4* st[2];
+    float4* ro[3];
+    float4* rd[3];
+    float4* lam[3];
+    float4* beta[3];
+    uint4* st[3];
     float2* hit;
     float4* co;      // connection origin.xyz, segment end
     float4* cd;      // connection dir.xyz, pixel index
@@ -34,6 +37,12 @@ struct Queues {
                      // occupancy its registers allow)
     int connUseSpill;
     int stats;       // 1: k_extend / k_connect count traversal statistics per job
+    int binned;      // 1: ray binning on (the binning kernels run and queue 2 is read)
+    int binCols;     // histogram columns (a multiple of kBinColStep, <= kBinCols)
+    unsigned* binHist;    // kBinCols x kBins counts, column-major by workgroup; the scan
+                          // turns them into each (column, bin)'s first slot
+    unsigned* binChunk;   // kBins x kBinChunks exclusive prefixes of the column chunks
+    unsigned* binTotal;   // kBins totals, then (after the scan) kBins first slots
     unsigned segCap, connSegCap, mirrorSegCap;
 };
 
@@ -47,6 +56,9 @@ void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s);
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s);
+// Ray binning of queue `cur` into queue 2 (four kernels: count, chunk sums, bin scan,
+// scatter).
+void launchBin(const Queues& q, int cur, hipStream_t s);
 void launchMirror(const Queues& q, const LaunchShape& ls, hipStream_t s);
 
 // Batch probes over host-provided rays (already on the device).

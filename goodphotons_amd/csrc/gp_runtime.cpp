@@ -304,6 +304,10 @@ gp_status fillJob(const gp_scene* s, const gp_camera_desc* cams, gp_film* const*
     J.heroC = p.hero_c;
     J.maxDepth = p.max_depth;
     J.rrDepth = p.rr_depth;
+    for (int a = 0; a < 3; ++a) {
+        J.binLo[a] = s->binLo[a];
+        J.binScale[a] = s->binScale[a];
+    }
     return GP_OK;
 }
 
@@ -320,6 +324,7 @@ gp_tuning defaultTuning() {
     t.use_graph = 1;
     t.lds_stack = 16;
     t.profile = 0;
+    t.ray_sort = 0;
     return t;
 }
 
@@ -441,6 +446,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
     const bool wantMirror = mirrors || mirrorQueue;
     const bool needMirrorAlloc = wantMirror && (needQueues || !q.mv[0]);
+    const int binned = tun.ray_sort ? 1 : 0;
+    const bool needBinAlloc = binned && (needQueues || !q.ro[2]);
     // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
     // spill[sp - SD], and sp < stackNeed). k_connect's stack holds 4-byte entries, so it
     // keeps at least 24 in LDS at no cost in occupancy (its registers already allow
@@ -452,8 +459,9 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     // The kernels without spill code keep the whole stack in LDS: they may only run
     // when every traced scene fits. A change of variant re-captures the graph.
     const int useSpill = needSpill > 0 ? 1 : 0;
-    if (!needQueues && !needSpillAlloc && !needMirrorAlloc && q.lds == lds && q.useSpill == useSpill &&
-        q.connLds == connLds && q.connUseSpill == connUseSpill && graphShape == ls.blocksExtend)
+    if (!needQueues && !needSpillAlloc && !needMirrorAlloc && !needBinAlloc && q.lds == lds &&
+        q.useSpill == useSpill && q.connLds == connLds && q.connUseSpill == connUseSpill &&
+        graphShape == ls.blocksExtend && q.binned == binned)
         return GP_OK;
     if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
     if (needQueues) {
@@ -517,6 +525,31 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
         }
         mirrorQueue = true;
     }
+    if (needBinAlloc) {
+        // the binned path queue (same slots as the others) and the counting-sort tables
+        const size_t pcap = (size_t)kSegs * segCap;
+        void* p[8] = {};
+        const size_t bytes[8] = {pcap * sizeof(float4), pcap * sizeof(float4), pcap * sizeof(float4),
+                                 pcap * sizeof(float4), pcap * sizeof(uint4),
+                                 (size_t)kBinCols * kBins * sizeof(unsigned),
+                                 (size_t)kBinChunks * kBins * sizeof(unsigned), (size_t)kBins * sizeof(unsigned)};
+        for (int k = 0; k < 8; ++k) {
+            hipError_t e = hipMalloc(&p[k], bytes[k]);
+            if (e != hipSuccess) {
+                for (int i = 0; i < k; ++i) hipFree(p[i]);
+                return fail(GP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+        }
+        for (int k = 0; k < 8; ++k) bufs[nbufs++] = p[k];
+        q.ro[2] = (float4*)p[0];
+        q.rd[2] = (float4*)p[1];
+        q.lam[2] = (float4*)p[2];
+        q.beta[2] = (float4*)p[3];
+        q.st[2] = (uint4*)p[4];
+        q.binHist = (unsigned*)p[5];
+        q.binChunk = (unsigned*)p[6];
+        q.binTotal = (unsigned*)p[7];
+    }
     if (needSpillAlloc) {
         const int depthAlloc = std::max(needSpill, spillDepth);
         if (q.spill) hipFree(q.spill);
@@ -531,6 +564,9 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     q.useSpill = useSpill;
     q.connLds = connLds;
     q.connUseSpill = connUseSpill;
+    q.binned = binned;
+    q.binCols = std::min(kBinCols, std::max(kBinColStep, (cus * 4) / kBinColStep * kBinColStep));
+    static_assert(kBinColStep % kBinChunks == 0 && kBinCols % kBinColStep == 0, "binning column shape");
     graphShape = ls.blocksExtend;
     return GP_OK;
 }
@@ -554,6 +590,8 @@ gp_status Engine::enqueueIterations(int n) {
         const int cur = parity;
         if ((st = timed(0, [&] { launchSchedule(q, cur, s); })) != GP_OK) return st;
         if ((st = timed(1, [&] { launchEmit(q, cur, ls, s); })) != GP_OK) return st;
+        if (q.binned)
+            if ((st = timed(5, [&] { launchBin(q, cur, s); })) != GP_OK) return st;
         if ((st = timed(2, [&] { launchExtend(q, cur, ls, s); })) != GP_OK) return st;
         if ((st = timed(3, [&] { launchShade(q, cur, ls, s); })) != GP_OK) return st;
         if ((st = timed(4, [&] { launchConnect(q, ls, s); })) != GP_OK) return st;
@@ -566,7 +604,7 @@ gp_status Engine::enqueueIterations(int n) {
 }
 
 void Engine::resetProfile() {
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kKernelSlots; ++k) {
         for (auto& ev : pending[k]) { hipEventDestroy(ev[0]); hipEventDestroy(ev[1]); }
         pending[k].clear();
         kernelMs[k] = 0.0;
@@ -575,7 +613,7 @@ void Engine::resetProfile() {
 }
 
 gp_status Engine::collectProfile() {
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kKernelSlots; ++k) {
         for (auto& ev : pending[k]) {
             float ms = 0.f;
             GP_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -593,7 +631,7 @@ gp_status Engine::runIterations(int n) {
     hipStream_t s = device().stream;
     const int K = chunkIterations(tun);
     if (!tun.use_graph || tun.profile || q.stats || parity != 0 || n < K) return enqueueIterations(n);
-    if (!graph || graphIters != K || graphLds != q.lds || graphMirror != mirrorActive) {
+    if (!graph || graphIters != K || graphLds != q.lds || graphMirror != mirrorActive || graphBinned != q.binned) {
         if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
         hipGraph_t g;
         GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -609,6 +647,7 @@ gp_status Engine::runIterations(int n) {
         graphIters = K;
         graphLds = q.lds;
         graphMirror = mirrorActive;
+        graphBinned = q.binned;
     }
     int done = 0;
     while (n - done >= K) {

@@ -18,6 +18,9 @@ struct gp_scene {
     int64_t nNodes = 0;                // four-wide nodes on the device
     int depth = 0;                     // child-pair levels of the built tree
     int stackNeed = 0;                 // traversal stack entries a ray can need (3 per four-wide level)
+    // ray-binning grid over the triangle-centroid box: cell = (x - binLo) * binScale
+    float binLo[3] = {0.f, 0.f, 0.f};
+    float binScale[3] = {0.f, 0.f, 0.f};
     float4* dNodes = nullptr;
     float4* dTris = nullptr;
     float4* dTrin = nullptr;
@@ -98,6 +101,9 @@ bool cameraBasis(const gp_camera_desc& c, CamBasis& out);
 // Smallest distance from point p to any triangle or emitter quad of the scene.
 double sceneMinDistance(const gp_scene& s, const double p[3]);
 
+// Kernel timing slots: schedule, emit, extend, shade, connect (+ mirror), binning.
+constexpr int kKernelSlots = 6;
+
 // Wavefront engine: device queues, control block, job table, graphs.
 struct Engine {
     std::mutex mu;
@@ -121,13 +127,14 @@ struct Engine {
     // ledger[5] and counters[kAccCounters] (the quantities from kAccPhotons on)
     gp_status readJobAcc(int slot, unsigned long long ledger[5], unsigned long long* counters);
     void initCtrl();                            // zero the host control block, set capacities
-    void* bufs[24] = {};
+    void* bufs[40] = {};
     int nbufs = 0;
     int parity = 0;
     hipGraphExec_t graph = nullptr;
     int graphIters = 0;
     int graphLds = 0;
     int graphShape = 0;
+    int graphBinned = 0;
     // k_mirror runs in each iteration while a job of a scene with flat mirrors is live
     // (set by the caller before runIterations; the graph is re-captured on a change)
     bool mirrorActive = false;
@@ -139,10 +146,10 @@ struct Engine {
     // step and starts from a fresh control block.
     const void* owner = nullptr;
     uint64_t epoch = 0;
-    // per-kernel HIP-event timing (tuning.profile)
-    std::vector<std::array<hipEvent_t, 2>> pending[5];
-    double kernelMs[5] = {};
-    uint64_t kernelLaunches[5] = {};
+    // per-kernel HIP-event timing (tuning.profile), slots as gp_trace_stats.kernel_ms
+    std::vector<std::array<hipEvent_t, 2>> pending[kKernelSlots];
+    double kernelMs[kKernelSlots] = {};
+    uint64_t kernelLaunches[kKernelSlots] = {};
     void resetProfile();
     gp_status collectProfile();
 

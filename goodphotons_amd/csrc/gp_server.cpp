@@ -577,6 +577,23 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
         bool armed = true;
         ~Restore() { if (armed) gp_set_tuning(&t); }
     } restore{base};
+    auto trial = [&](const gp_tuning& t) -> gp_status {
+        gp_status s0 = gp_set_tuning(&t);
+        if (s0 != GP_OK) return s0;
+        gp_trace_params p = *params;
+        p.n_photons = std::min<int64_t>(params->n_photons, std::max<int64_t>(1 << 21, 2 * (int64_t)t.queue_capacity));
+        gp_film* film = nullptr;
+        if ((s0 = gp_film_create(cam->res_x, cam->res_y, 2 * p.n_photons, &film)) != GP_OK) return s0;
+        gp_energy e{};
+        gp_trace_stats s{};
+        s0 = gp_trace(scene, cam, film, &p, &e, &s);        // warm (allocations, graph)
+        if (s0 == GP_OK) s0 = gp_trace(scene, cam, film, &p, &e, &s);
+        gp_film_destroy(film);
+        if (s0 != GP_OK) return s0;
+        const double rate = (double)s.segments / std::max(1e-6, s.wall_ms);
+        if (rate > bestRate) { bestRate = rate; bestT = t; }
+        return GP_OK;
+    };
     for (int cap : caps)
         for (int w : waves)
             for (int l : ldss) {
@@ -584,20 +601,14 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
                 t.queue_capacity = cap;
                 t.grid_waves_per_cu = w;
                 t.lds_stack = l;
-                if ((st = gp_set_tuning(&t)) != GP_OK) return st;
-                gp_trace_params p = *params;
-                p.n_photons = std::min<int64_t>(params->n_photons, std::max<int64_t>(1 << 21, 2 * (int64_t)cap));
-                gp_film* film = nullptr;
-                if ((st = gp_film_create(cam->res_x, cam->res_y, 2 * p.n_photons, &film)) != GP_OK) return st;
-                gp_energy e{};
-                gp_trace_stats s{};
-                st = gp_trace(scene, cam, film, &p, &e, &s);        // warm (allocations, graph)
-                if (st == GP_OK) st = gp_trace(scene, cam, film, &p, &e, &s);
-                gp_film_destroy(film);
-                if (st != GP_OK) return st;
-                const double rate = (double)s.segments / std::max(1e-6, s.wall_ms);
-                if (rate > bestRate) { bestRate = rate; bestT = t; }
+                if ((st = trial(t)) != GP_OK) return st;
             }
+    // then ray binning on or off at the best shape
+    {
+        gp_tuning t = bestT;
+        t.ray_sort = bestT.ray_sort ? 0 : 1;
+        if ((st = trial(t)) != GP_OK) return st;
+    }
     if ((st = gp_set_tuning(&bestT)) != GP_OK) return st;
     restore.armed = false;
     if (best) *best = bestT;

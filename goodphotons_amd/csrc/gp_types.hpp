@@ -44,6 +44,20 @@ constexpr int kAccAnyNodeVisits = 12; // camera shadow rays' node visits (traver
 constexpr int kAccAnyTriTests = 13;   // camera shadow rays' triangle tests (traversal statistics only)
 constexpr int kAccCounters = 9;       // counters [kAccPhotons, kAccPhotons + kAccCounters)
 
+// Ray binning (gp_tuning.ray_sort): before the closest-hit kernel, the paths of an
+// iteration are reordered by a 9-bit key, the Morton code of the origin's cell in a
+// 4 x 4 x 4 grid over the job's triangle-centroid box followed by the direction
+// octant, so that the lanes of a wave trace similar rays.
+// A counting sort: binCols workgroups (4 per CU, at most kBinCols) each count their
+// contiguous share of the queue into a histogram column, the columns are scanned in
+// (bin, column) order, and each workgroup scatters its share to the bins' ranges.
+constexpr int kBinBits = 9;
+constexpr int kBins = 1 << kBinBits;
+constexpr int kBinCols = 1024;        // most histogram columns (workgroups of the count/scatter)
+constexpr int kBinChunks = 8;         // column chunks per bin in the scan
+constexpr int kBinColStep = 8;        // the column count is a multiple of this (and of kBinChunks)
+constexpr int kBinCellsPerAxis = 4;
+
 // Leaf links are negative: ~((first << 3) | (count - 1)).
 inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (count - 1)); }
 
@@ -120,6 +134,8 @@ struct DevJob {
     // path settings
     unsigned long long seedBase, salt;
     int32_t heroC, maxDepth, rrDepth, pad0;
+    // ray binning: origin cell = (o - binLo) * binScale, clamped to the grid
+    float binLo[3], binScale[3];
 };
 
 // Connection tag (float bits of the record's 4th word): job slot in bits 0..15,

@@ -206,9 +206,10 @@ typedef struct gp_trace_stats {
     uint64_t iterations;     /* wavefront iterations run                             */
     double wall_ms;          /* host wall time of the trace loop                     */
     /* With gp_tuning.profile = 1: summed device time and launch count per kernel,
-     * index 0 schedule, 1 emit, 2 extend (closest hit), 3 shade, 4 connect. */
-    double kernel_ms[5];
-    uint64_t kernel_launches[5];
+     * index 0 schedule, 1 emit, 2 extend (closest hit), 3 shade, 4 connect (and the
+     * flat-mirror connections), 5 ray binning (its four kernels, gp_tuning.ray_sort). */
+    double kernel_ms[6];
+    uint64_t kernel_launches[6];
     /* With GP_TRACE_TRAVERSAL_STATS (else 0), over the four-wide tree: nodes loaded and
      * slab-tested (a leaf is a link of its parent, not a node) and triangles
      * intersected by the segments' closest hits, and the same for the camera shadow
@@ -391,6 +392,12 @@ typedef struct gp_tuning {
     int32_t use_graph;       /* 1 = replay captured graphs, 0 = plain launches       */
     int32_t lds_stack;       /* traversal stack entries per lane held in LDS (8..32)  */
     int32_t profile;         /* 1 = time every kernel launch with HIP events (no graphs) */
+    int32_t ray_sort;        /* 1 = reorder each iteration's paths by origin cell
+                                (4 x 4 x 4 over the scene) and direction octant before
+                                the closest-hit kernel, a counting sort into a third
+                                path queue, so that the lanes of a wave trace similar
+                                rays; 0 = queue order (default). Results are
+                                identical either way; gp_autotune tries both. */
 } gp_tuning;
 
 gp_status gp_get_tuning(gp_tuning* out);

@@ -12,6 +12,8 @@ memory access of the device code is bounds-checked without a GPU.
 - Regression for the splat-table race in k_connect: at 128^2 with a 2^14-path queue on
   a 4-workgroup grid, the old table re-insertion (no barrier after it) indexed the LDS
   table with -1 and sent a splat to a garbage film address here; it must run clean.
+- Ray binning (the counting sort of each iteration's paths into a third queue, and
+  the closest-hit and shading kernels reading it): bit-identical to the unbinned trace.
 - The device BVH builder (linear BVH and its collapse into four-wide nodes) with the
   four-wide traversal: closest-hit prim ids against the oracle on edge-case scenes and
   a soup, and c0 films identical with the host-built and the device-built tree.
@@ -40,15 +42,16 @@ def _env(cus, preempt=0):
     return env
 
 
-def _trace(name, cap, n, res, lds=16):
+def _trace(name, cap, n, res, lds=16, ray_sort=0):
     return [sys.executable, os.path.join(ROOT, "tests", "emu", "run_trace.py"), name, str(cap), str(n), str(res),
-            str(lds)]
+            str(lds), str(ray_sort)]
 
 
 # Every emulated run is one mostly single-threaded child process (one lane runs at a
 # time), so the module starts them all at once and each test waits for its own.
 CASES = {
     "c1_interleaved": (_trace("c1", 4096, 4096, 32), _env(2, preempt=3)),
+    "c1_binned": (_trace("c1", 4096, 4096, 32, 16, ray_sort=1), _env(2, preempt=3)),
     "mirror_lds": (_trace("mirror", 4096, 2048, 32, 16), _env(2, preempt=3)),
     "mirror_spill": (_trace("mirror", 4096, 2048, 32, 8), _env(2, preempt=3)),
     "splat_flush": (_trace("c0", 1 << 14, 1 << 14, 128), _env(1)),
@@ -91,6 +94,17 @@ def test_interleaved_lanes_trace_the_oracles_photons(runs, orc):
     assert (st["segments"], st["connections"], st["splats"]) == (cst["segments"], cst["connections"], cst["splats"])
     m = image_metrics(x, cx, 4096, 4096, res=(32, 32))
     assert m["rel_rmse_max"] < 1e-5 and np.array_equal(h, ch)
+
+
+def test_ray_binning_changes_nothing(runs, orc):
+    # the binning kernels (count, scan, scatter into the third queue) and the binned
+    # reads of extend/shade, lanes interleaved: the same photons, bit for bit
+    x, h, e, st = _film(runs, "c1_binned")
+    px, ph, pe, pst = _film(runs, "c1_interleaved")
+    assert np.array_equal(x, px) and np.array_equal(h, ph) and e == pe
+    assert (st["segments"], st["connections"], st["splats"]) == (pst["segments"], pst["connections"], pst["splats"])
+    cx, ch, ce, cst = _oracle(orc, "c1", 4096, 32)
+    assert st["segments"] == cst["segments"] and np.array_equal(h, ch)
 
 
 @pytest.mark.parametrize("key", ["mirror_lds", "mirror_spill"], ids=["lds_stack", "spilling_stack"])
