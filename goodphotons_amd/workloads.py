@@ -185,10 +185,11 @@ def _run_plain(lib, sc, jobs, arrivals, tuning=None):
             lib.set_tuning(**saved)
 
 
-def _run_server(lib, sc, jobs, arrivals, order_policy=gp.abi.ORDER_SHORTEST_FIRST):
+def _run_server(lib, sc, jobs, arrivals, order_policy=gp.abi.ORDER_SHORTEST_FIRST, max_live_jobs=64):
     """Continuous batching: submit on arrival, keep the device fed, take each result
-    (film downloaded) as soon as it finishes."""
-    srv = gp.Server(hbm_budget_bytes=8 << 30, max_live_jobs=64, lib=lib, order_policy=order_policy)
+    (film downloaded) as soon as it finishes. max_live_jobs = 1 runs one job at a time
+    through the same loop (no overlap of one job's drain with the next job's photons)."""
+    srv = gp.Server(hbm_budget_bytes=8 << 30, max_live_jobs=max_live_jobs, lib=lib, order_policy=order_policy)
     try:
         t0 = time.perf_counter()
         live = {}
@@ -232,6 +233,7 @@ def run_c4(n_jobs=60, lam=20.0, full=True):
     for mode, arrivals in (("poisson", [j["arrival"] for j in jobs]), ("saturated", [0.0] * len(jobs))):
         r = {"server": _run_server(lib, sc, jobs, arrivals),
              "server_admission_order": _run_server(lib, sc, jobs, arrivals, gp.abi.ORDER_ADMISSION),
+             "server_one_live_job": _run_server(lib, sc, jobs, arrivals, gp.abi.ORDER_ADMISSION, 1),
              "plain_reference_style": _run_plain(lib, sc, jobs, arrivals, REFERENCE_STYLE_TUNING),
              "plain_tuned": _run_plain(lib, sc, jobs, arrivals)}
         r["throughput_ratio_vs_reference_style"] = (r["server"]["segments_per_s"] /
