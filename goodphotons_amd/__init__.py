@@ -53,6 +53,9 @@ class Library:
         L.gp_film_reset.argtypes = [vp]
         L.gp_film_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gp_film_download_raw.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.gp_film_download_hits.argtypes = [vp, C.POINTER(C.c_double)]
+        L.gp_film_seed.argtypes = [vp, vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.TraceParams),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gp_film_destroy.argtypes = [vp]
         L.gp_film_destroy.restype = None
         L.gp_film_to_linear.argtypes = [vp, C.c_int64, C.POINTER(C.c_float)]
@@ -205,6 +208,21 @@ class Film:
         self.L.check(self.L.lib.gp_film_download_raw(self.h, w.ctypes.data_as(C.POINTER(C.c_int64)),
                                                      C.byref(sc)))
         return w, sc.value
+
+    def download_hits(self):
+        """Hit counts only (gp_film_download_hits)."""
+        hits = np.zeros(self.rx * self.ry)
+        self.L.check(self.L.lib.gp_film_download_hits(self.h, hits.ctypes.data_as(C.POINTER(C.c_double))))
+        return hits
+
+    def seed(self, scene: "GpuScene", cam, n_photons, xyz, hits, max_depth=32, hero_c=4):
+        """gp_film_seed: running totals (doubles) of n_photons photons into this fresh film."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        h = np.ascontiguousarray(hits, np.float64).reshape(-1)
+        p = trace_params(n_photons, hero_c=hero_c, max_depth=max_depth)
+        self.L.check(self.L.lib.gp_film_seed(self.h, scene.h, C.byref(cam), C.byref(p),
+                                             x.ctypes.data_as(C.POINTER(C.c_double)),
+                                             h.ctypes.data_as(C.POINTER(C.c_double))))
 
     def reset(self):
         self.L.check(self.L.lib.gp_film_reset(self.h))

@@ -132,7 +132,8 @@ TRACE_TRAVERSAL_STATS = 1           # gp_trace_params.flags: GP_TRACE_TRAVERSAL_
 EXPORTED_SYMBOLS = [
     "gp_abi_version", "gp_status_string", "gp_last_error", "gp_device_query",
     "gp_scene_create", "gp_scene_create_ex", "gp_scene_get_info", "gp_scene_destroy", "gp_mirror_planes",
-    "gp_film_create", "gp_film_reset", "gp_film_download", "gp_film_download_raw",
+    "gp_film_create", "gp_film_reset", "gp_film_download", "gp_film_download_raw", "gp_film_download_hits",
+    "gp_film_seed",
     "gp_film_destroy", "gp_film_to_linear", "gp_film_tonemap", "gp_film_tonemap_noalloc",
     "gp_checkpoint_guard", "gp_film_save_checkpoint", "gp_film_load_checkpoint",
     "gp_trace", "gp_trace_shared", "gp_probe_closest_hit", "gp_probe_any_hit",

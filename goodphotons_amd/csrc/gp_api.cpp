@@ -489,6 +489,56 @@ gp_status gp_film_download(gp_film* f, double* xyz, double* hits) {
     return GP_OK;
 }
 
+gp_status gp_film_download_hits(gp_film* f, double* hits) {
+    if (!f || !hits) return fail(GP_ERR_INVALID, "null argument");
+    const size_t np = (size_t)f->rx * f->ry;
+    std::vector<int64_t> h(np);
+    // the fourth word of every pixel: a strided copy of 8 of each 32 bytes
+    GP_HIP(hipMemcpy2D(h.data(), sizeof(int64_t), f->d + 3, 4 * sizeof(int64_t), sizeof(int64_t), np,
+                       hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < np; ++i) hits[i] = (double)h[i];
+    return GP_OK;
+}
+
+gp_status gp_film_seed(gp_film* f, gp_scene* s, const gp_camera_desc* cam, const gp_trace_params* p,
+                       const double* xyz, const double* hits) {
+    if (!f || !s || !cam || !p || !xyz || !hits) return fail(GP_ERR_INVALID, "null argument");
+    if (f->scale > 0.0) return fail(GP_ERR_STATE, "film already holds a trace (seed a fresh film)");
+    if (p->n_photons < 0 || p->max_depth < 1) return fail(GP_ERR_INVALID, "invalid trace parameters");
+    if (cam->res_x != f->rx || cam->res_y != f->ry) return fail(GP_ERR_INVALID, "camera resolution differs from the film's");
+    CamBasis cb;
+    if (!cameraBasis(*cam, cb)) return fail(GP_ERR_INVALID, "degenerate camera");
+    const size_t np = (size_t)f->rx * f->ry;
+    // check everything before the film's scale is fixed
+    const double lim = std::ldexp(1.0, 62);
+    for (size_t i = 0; i < np; ++i) {
+        if (!(hits[i] >= 0.0 && hits[i] < lim && hits[i] == std::floor(hits[i])))
+            return fail(GP_ERR_INVALID, "seed hit counts must be whole numbers in [0, 2^62)");
+        for (int c = 0; c < 3; ++c)
+            if (!(xyz[3 * i + c] >= 0.0 && std::isfinite(xyz[3 * i + c])))
+                return fail(GP_ERR_INVALID, "seed sums must be finite and non-negative");
+    }
+    const double sScale = f->scale, sBudget = f->splatBudget, sUsed = f->splatUsed;
+    const unsigned long long sQmax = f->qmax;
+    auto restore = [&] {
+        f->scale = sScale; f->qmax = sQmax; f->splatBudget = sBudget; f->splatUsed = sUsed;
+    };
+    gp_status st = prepareFilm(f, s, cb, *p);
+    if (st != GP_OK) { restore(); return st; }
+    std::vector<int64_t> w(np * 4);
+    for (size_t i = 0; i < np; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const double q = std::nearbyint(xyz[3 * i + c] * f->scale);
+            if (!(q < lim)) { restore(); return fail(GP_ERR_INVALID, "seed sum beyond the film's range"); }
+            w[4 * i + c] = (int64_t)q;
+        }
+        w[4 * i + 3] = (int64_t)hits[i];
+    }
+    hipError_t e = hipMemcpy(f->d, w.data(), w.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { restore(); return fail(GP_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)); }
+    return GP_OK;
+}
+
 void gp_film_destroy(gp_film* f) { delete f; }
 
 // ------------------------------------------------------------------ trace

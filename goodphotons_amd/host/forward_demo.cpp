@@ -1,8 +1,10 @@
 // C++ caller of the forward core through gp_forward.hpp: renders an empty Cornell
 // box (5 wall quads + ceiling light quad) with renderForwardGpu, then again through
 // a ForwardSession in two batches, and once more checkpointed after the first batch
-// and resumed in a second session; writes the films and ledgers as raw
-// doubles. Usage: forward_demo <out-prefix> <res> <photons>
+// and resumed in a second session; then a two-camera SharedSession in two batches,
+// and again stopped after the first and resumed from the downloaded totals (seeded
+// films); writes the films and ledgers as raw doubles and prints the session checks.
+// Usage: forward_demo <out-prefix> <res> <photons>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -104,6 +106,38 @@ int main(int argc, char** argv) {
     gp::EnergyReport e5;
     std::vector<gp::Film> fs = gp::renderForwardSharedGpu(s, {cam, cam2}, {res, res / 2}, {res, res / 2}, N, e5,
                                                          true, 'B', 0, true, 4);
+    // resident two-camera session: uninterrupted, and resumed from downloaded totals
+    std::vector<gp::Film> g1, g2, seed;
+    gp::EnergyReport eg1, eg2, eseed;
+    std::vector<double> poll;
+    {
+        gp::SharedSession ss;
+        if (!ss.begin(s, {cam, cam2}, {res, res / 2}, {res, res / 2}, 'B', 4, 4 * N)) return 1;
+        if (!ss.batch(N / 2, 0) || !ss.batch(N - N / 2, (unsigned long long)(N / 2)) || !ss.hits0(poll) ||
+            !ss.download(g1, eg1))
+            return 1;
+    }
+    {
+        gp::SharedSession ss;
+        if (!ss.begin(s, {cam, cam2}, {res, res / 2}, {res, res / 2}, 'B', 4, 4 * N) || !ss.batch(N / 2, 0) ||
+            !ss.download(seed, eseed))
+            return 1;
+    }
+    {
+        gp::SharedSession ss;
+        if (!ss.begin(s, {cam, cam2}, {res, res / 2}, {res, res / 2}, 'B', 4, 4 * N, &seed, &eseed, N / 2) ||
+            !ss.batch(N - N / 2, (unsigned long long)(N / 2)) || !ss.download(g2, eg2))
+            return 1;
+    }
+    bool same = g1.size() == 2 && g2.size() == 2;
+    for (size_t c = 0; same && c < 2; ++c) {
+        same = g1[c].hits == g2[c].hits;
+        for (size_t i = 0; same && i < g1[c].xyz.size(); ++i)
+            same = g1[c].xyz[i].x == g2[c].xyz[i].x && g1[c].xyz[i].y == g2[c].xyz[i].y && g1[c].xyz[i].z == g2[c].xyz[i].z;
+    }
+    std::printf("shared_session_resume_exact %d\n", (int)same);
+    std::printf("shared_session_hits0_equal %d\n", (int)(g1.size() == 2 && poll == g1[0].hits));
+    std::printf("shared_session_hits_equal_single %d\n", (int)(g1.size() == 2 && g1[0].hits == f1.hits));
     gp::EnergyReport e3;
     gp::Film f3 = gp::renderForwardGpu(s, cam, res, res, N, e3, true, 'A');   // unsupported model
     double sum3 = 0;

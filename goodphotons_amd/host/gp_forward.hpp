@@ -14,6 +14,11 @@
 //   ForwardSession (begin / batch / download / end)
 //       <- sharedForwardGpuBegin / Batch / Download / End (src/render_cuda.h:121-143),
 //          one camera: scene and film resident in HBM between batches.
+//   SharedSession (begin / batch / hits0 / download / end)
+//       <- the same session with 1..16 cameras sharing each batch's photon flight,
+//          the camera-0 noise poll (sharedForwardGpuHits0) and the resume seed
+//          (begin's seedFilms / seedEnergy). SessionCore holds its device state for
+//          any host types (ref_adapter.hpp wraps it for the reference's).
 //
 // The scene types mirror the subset of the reference's Scene the core supports
 // (triangles, Diffuse / Dielectric / Mirror / Glossy materials with analytic or sampled
@@ -242,6 +247,146 @@ inline std::vector<Film> renderForwardSharedGpu(const Scene& scene, const std::v
 
 // Resident forward session (one camera): the scene and film stay in HBM; every
 // batch traces N more photons into the running totals.
+// Device state of a resident multi-camera session, independent of the host types:
+// the scene, one film per camera (fixed-point, sized for maxPhotons photons in all)
+// and the running ledger.
+class SessionCore {
+public:
+    bool open(const gp_scene_desc& d, int32_t builder, const std::vector<gp_camera_desc>& cams,
+              long long maxPhotons) {
+        close();
+        if (cams.empty() || cams.size() > 16) { std::fprintf(stderr, "[goodphotons] session: 1..16 cameras\n"); return false; }
+        if (gp_scene_create_ex(&d, builder, &scene_) != GP_OK) { detail::report("session scene"); return false; }
+        cams_ = cams;
+        films_.assign(cams.size(), nullptr);
+        for (size_t c = 0; c < cams.size(); ++c)
+            if (gp_film_create(cams[c].res_x, cams[c].res_y, maxPhotons, &films_[c]) != GP_OK) {
+                detail::report("session film");
+                close();
+                return false;
+            }
+        return true;
+    }
+    // Running totals of camera c (pixel-major doubles) from an earlier session or
+    // checkpoint: seedPhotons photons traced with maxDepth.
+    bool seed(size_t c, const double* xyz, const double* hits, long long seedPhotons, int maxDepth) {
+        gp_trace_params p{};
+        p.n_photons = seedPhotons;
+        p.hero_c = 1;
+        p.max_depth = maxDepth;
+        if (gp_film_seed(films_[c], scene_, &cams_[c], &p, xyz, hits) != GP_OK) { detail::report("session seed"); return false; }
+        return true;
+    }
+    void seedEnergy(const gp_energy& e) { energy_ = e; }
+    bool batch(long long N, unsigned long long seedBase, uint64_t salt, int heroC, int maxDepth, int rrDepth) {
+        gp_trace_params p{};
+        p.n_photons = N;
+        p.seed_base = seedBase;
+        p.salt = salt;
+        p.hero_c = heroC < 1 ? 1 : (heroC > 4 ? 4 : heroC);
+        p.max_depth = maxDepth;
+        p.rr_depth = rrDepth;
+        if (gp_trace_shared(scene_, (int32_t)films_.size(), cams_.data(), films_.data(), &p, &energy_, nullptr) != GP_OK) {
+            detail::report("session batch");
+            return false;
+        }
+        return true;
+    }
+    bool hits0(double* hits) const { return gp_film_download_hits(films_[0], hits) == GP_OK; }
+    bool download(size_t c, double* xyz, double* hits) const { return gp_film_download(films_[c], xyz, hits) == GP_OK; }
+    const gp_energy& energy() const { return energy_; }
+    size_t cameras() const { return films_.size(); }
+    const gp_camera_desc& camera(size_t c) const { return cams_[c]; }
+    double totalPower() const {
+        gp_scene_info i{};
+        return gp_scene_get_info(scene_, &i) == GP_OK ? i.total_power : 0.0;
+    }
+    bool isOpen() const { return scene_ != nullptr; }
+    void close() {
+        for (gp_film* f : films_)
+            if (f) gp_film_destroy(f);
+        films_.clear();
+        if (scene_) gp_scene_destroy(scene_);
+        scene_ = nullptr;
+        energy_ = gp_energy{};
+    }
+    ~SessionCore() { close(); }
+
+private:
+    gp_scene* scene_ = nullptr;
+    std::vector<gp_film*> films_;
+    std::vector<gp_camera_desc> cams_;
+    gp_energy energy_{};
+};
+
+// Resident shared session for the core's own types, as sharedForwardGpuBegin / Batch /
+// Hits0 / Download / End. maxPhotons bounds the photons the session may hold in all
+// (seed included): it fixes the films' fixed-point scale.
+class SharedSession {
+public:
+    bool begin(const Scene& scene, const std::vector<Camera>& cams, const std::vector<int>& resX,
+               const std::vector<int>& resY, char camMode, int heroC, long long maxPhotons,
+               const std::vector<Film>* seedFilms = nullptr, const EnergyReport* seedEnergy = nullptr,
+               long long seedPhotons = 0) {
+        if (camMode != 'B' || cams.size() != resX.size() || cams.size() != resY.size()) return false;
+        std::vector<float> v;
+        std::vector<int32_t> m;
+        const gp_scene_desc d = detail::describe(scene, v, m);
+        std::vector<gp_camera_desc> cd;
+        for (size_t c = 0; c < cams.size(); ++c) cd.push_back(detail::camera(cams[c], resX[c], resY[c]));
+        if (!core_.open(d, GP_BVH_HOST_SAH, cd, maxPhotons)) return false;
+        heroC_ = heroC;
+        if (seedFilms) {
+            if (seedFilms->size() != cams.size()) { core_.close(); return false; }
+            for (size_t c = 0; c < cams.size(); ++c) {
+                const Film& f = (*seedFilms)[c];
+                std::vector<double> xyz((size_t)resX[c] * resY[c] * 3);
+                for (size_t i = 0; i < f.xyz.size() && 3 * i + 2 < xyz.size(); ++i) {
+                    xyz[3 * i] = f.xyz[i].x; xyz[3 * i + 1] = f.xyz[i].y; xyz[3 * i + 2] = f.xyz[i].z;
+                }
+                if (f.hits.size() != (size_t)resX[c] * resY[c] ||
+                    !core_.seed(c, xyz.data(), f.hits.data(), seedPhotons, gMaxDepth)) {
+                    core_.close();
+                    return false;
+                }
+            }
+        }
+        if (seedEnergy)
+            core_.seedEnergy(gp_energy{seedEnergy->emitted, seedEnergy->absorbed, seedEnergy->sensor,
+                                       seedEnergy->escaped, seedEnergy->residual});
+        return true;
+    }
+    bool batch(long long N, unsigned long long seedBase) {
+        return core_.batch(N, seedBase, kReferenceSalt, heroC_, gMaxDepth, gRrDepth);
+    }
+    // Camera 0's running hit counts only (a quarter of its film's bytes).
+    bool hits0(std::vector<double>& hits) {
+        hits.resize((size_t)core_.camera(0).res_x * core_.camera(0).res_y);
+        return core_.hits0(hits.data());
+    }
+    // Running totals, seed included (REPLACE the caller's accumulators).
+    bool download(std::vector<Film>& films, EnergyReport& e) {
+        films.resize(core_.cameras());
+        for (size_t c = 0; c < core_.cameras(); ++c) {
+            films[c].resX = core_.camera(c).res_x;
+            films[c].resY = core_.camera(c).res_y;
+            films[c].alloc();
+            std::vector<double> xyz(films[c].hits.size() * 3);
+            if (!core_.download(c, xyz.data(), films[c].hits.data())) return false;
+            for (size_t i = 0; i < films[c].xyz.size(); ++i)
+                films[c].xyz[i] = Vec3d{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        }
+        const gp_energy& g = core_.energy();
+        e = EnergyReport{g.emitted, g.absorbed, g.sensor, g.escaped, g.residual};
+        return true;
+    }
+    void end() { core_.close(); }
+
+private:
+    SessionCore core_;
+    int heroC_ = 4;
+};
+
 class ForwardSession {
 public:
     bool begin(const Scene& scene, const Camera& cam, int resX, int resY, int heroC,

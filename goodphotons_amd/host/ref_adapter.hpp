@@ -15,6 +15,11 @@
 //   cudaForwardSupported(scene)  (src/render_cuda.h:56)  -> gpref::forwardSupported
 //   cudaAvailable / cudaDeviceName / cudaMemInfo / cudaGracefulShutdown
 //                                (src/render_cuda.h:24-52) -> gpref::available ...
+//   sharedForwardGpuBegin / Batch / Hits0 / Download / End (src/render_cuda.h:117-143)
+//       -> gpref::SharedSession: scene and 1..16 camera films resident in HBM, each
+//          batch one shared photon flight, the camera-0 hit poll, running totals
+//          (seed included) on download; begin fails (the caller then stays on the
+//          CPU path) outside the supported set.
 //
 // What crosses the boundary (the role of buildUploadScene, src/render_cuda.cu:12692):
 //   - triangles: Scene::tris vertices (double -> float) and material ids, in order;
@@ -45,6 +50,7 @@
 #include <vector>
 
 #include "../../include/goodphotons.h"
+#include "gp_forward.hpp"
 
 namespace gpref {
 
@@ -273,5 +279,85 @@ inline Film renderForward(const Scene& scene, const Camera& cam, int resX, int r
     gp_scene_destroy(gs);
     return out;
 }
+
+// The reference's resident shared session (src/render_cuda.h:103-143). Its films
+// accumulate without a photon limit; the core's fixed-point films are sized for
+// gSessionMaxPhotons photons in all (2^36 by default: the largest single splat is then
+// 2^20 units of the film's scale at the reference's maxBounce of 32).
+inline long long gSessionMaxPhotons = 1LL << 36;
+
+class SharedSession {
+public:
+    // seedFilms / seedEnergy: the running totals of a resumed render (a checkpoint the
+    // reference wrote); downloads then return totals including them. The photons they
+    // hold, counted in the films' budget, follow from the emitted energy (each photon
+    // carries the scene's total power).
+    bool begin(const Scene& scene, const std::vector<Camera>& cams, const std::vector<int>& resX,
+               const std::vector<int>& resY, char camMode, bool wavefront, int heroC, bool beamGather,
+               const std::vector<Film>* seedFilms = nullptr, const EnergyReport* seedEnergy = nullptr) {
+        (void)wavefront;    // the core always runs as a wavefront
+        (void)beamGather;   // collimated-beam gathering is outside the supported set
+        if (camMode != 'B' || heroC < 1 || heroC > 4 || cams.empty() || cams.size() > 16 ||
+            resX.size() != cams.size() || resY.size() != cams.size())
+            return false;
+        Upload u;
+        if (!describe(scene, u)) return false;
+        std::vector<gp_camera_desc> cd;
+        for (size_t c = 0; c < cams.size(); ++c) cd.push_back(describeCamera(cams[c], resX[c], resY[c]));
+        if (!core_.open(u.desc, gBvh, cd, gSessionMaxPhotons)) return false;
+        heroC_ = heroC;
+        if (seedFilms) {
+            const double P = core_.totalPower();
+            const long long nSeed = (seedEnergy && P > 0.0) ? std::llround(seedEnergy->emitted / P) : 0;
+            if (seedFilms->size() != cams.size()) { core_.close(); return false; }
+            for (size_t c = 0; c < cams.size(); ++c) {
+                const Film& f = (*seedFilms)[c];
+                const size_t np = (size_t)resX[c] * resY[c];
+                if (f.hits.size() != np || f.xyz.size() != np) { core_.close(); return false; }
+                std::vector<double> xyz(np * 3);
+                for (size_t i = 0; i < np; ++i) {
+                    xyz[3 * i] = f.xyz[i].x; xyz[3 * i + 1] = f.xyz[i].y; xyz[3 * i + 2] = f.xyz[i].z;
+                }
+                if (!core_.seed(c, xyz.data(), f.hits.data(), nSeed, gMaxDepth)) { core_.close(); return false; }
+            }
+        }
+        if (seedEnergy)
+            core_.seedEnergy(gp_energy{seedEnergy->emitted, seedEnergy->absorbed, seedEnergy->sensor,
+                                       seedEnergy->escaped, seedEnergy->residual});
+        return true;
+    }
+    // N more photons (absolute indices seedBase ..) into every camera's film.
+    void batch(long long N, unsigned long long seedBase, bool diffraction) {
+        (void)diffraction;
+        core_.batch(N, seedBase, gSalt, heroC_, gMaxDepth, 0);
+    }
+    // Camera 0's running hit counts (hits must already hold its pixel count).
+    void hits0(std::vector<double>& hits) {
+        if (hits.size() != (size_t)core_.camera(0).res_x * core_.camera(0).res_y) return;
+        core_.hits0(hits.data());
+    }
+    // Running totals into films[c] (allocated at each camera's resolution); e is
+    // REPLACED with the total ledger.
+    void download(std::vector<Film>& films, EnergyReport& e) {
+        for (size_t c = 0; c < core_.cameras() && c < films.size(); ++c) {
+            const size_t np = (size_t)core_.camera(c).res_x * core_.camera(c).res_y;
+            if (films[c].hits.size() != np || films[c].xyz.size() != np) continue;
+            std::vector<double> xyz(np * 3);
+            if (!core_.download(c, xyz.data(), films[c].hits.data())) continue;
+            for (size_t i = 0; i < np; ++i) films[c].xyz[i] = Vec3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        }
+        const gp_energy& g = core_.energy();
+        e.emitted = g.emitted;
+        e.absorbed = g.absorbed;
+        e.sensor = g.sensor;
+        e.escaped = g.escaped;
+        e.residual = g.residual;
+    }
+    void end() { core_.close(); }
+
+private:
+    gp::SessionCore core_;
+    int heroC_ = 1;
+};
 
 }  // namespace gpref

@@ -299,6 +299,24 @@ gp_status gp_film_download(gp_film* film, double* xyz, double* hits);
 /* Raw fixed-point words, res_x*res_y*4 int64 (X, Y, Z, hits); scale_out gets the
  * factor that maps a double contribution to one integer unit. */
 gp_status gp_film_download_raw(gp_film* film, int64_t* words, double* scale_out);
+/* The hit counts alone, res_x*res_y doubles: a quarter of the film's bytes cross
+ * PCIe. Replaces sharedForwardGpuHits0 (src/render_cuda.h:135-137), the progressive
+ * loop's per-batch noise poll of camera 0. */
+gp_status gp_film_download_hits(gp_film* film, double* hits);
+/* Seeds a fresh film (nothing traced into it yet) with running totals held as
+ * doubles, such as a checkpoint the reference wrote: replaces the seedFilms of
+ * sharedForwardGpuBegin (src/render_cuda.h:117-130, src/render_cuda.cu:13806-13825).
+ * The film's fixed-point scale is fixed as a trace of params->n_photons photons
+ * (the photons the seed holds, counted in the film's budget) of `scene` seen by
+ * `cam` would fix it; each X, Y, Z sum is stored rounded to the nearest integer unit
+ * (an error of at most half a unit per pixel and channel, where the largest single
+ * splat is 2^K units, K >= 8) and each hit count exactly. Sums that are negative,
+ * not finite or beyond the film's 2^62 range, and hit counts that are not whole
+ * numbers, are refused with GP_ERR_INVALID; a film already traced into with
+ * GP_ERR_STATE. Sums downloaded from a film of the same scene, camera and
+ * max_photons come back to the same words, so tracing on continues bit-exactly. */
+gp_status gp_film_seed(gp_film* film, gp_scene* scene, const gp_camera_desc* cam,
+                       const gp_trace_params* params, const double* xyz, const double* hits);
 void gp_film_destroy(gp_film* film);
 
 /* ---- checkpoints (.ftbuf) ------------------------------------------------ */
@@ -317,7 +335,8 @@ gp_status gp_film_save_checkpoint(gp_film* film, const char* path, int64_t n_pho
 /* Recreate a device film from a checkpoint written by gp_film_save_checkpoint.
  * GP_ERR_STATE if mode or guard differ from the file's (readCheckpoint refuses the
  * same files); GP_ERR_UNSUPPORTED for a file without the fixed-point section (one
- * the reference wrote: resume it on the host, as readCheckpoint + Film::merge do).
+ * the reference wrote: read it with readCheckpoint and seed a film with
+ * gp_film_seed).
  * n_photons_out: the cumulative photon count, i.e. the seed_base of the next trace. */
 gp_status gp_film_load_checkpoint(const char* path, int32_t mode, uint64_t guard, gp_film** film_out,
                                   int64_t* n_photons_out, gp_energy* energy_out);

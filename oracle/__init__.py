@@ -422,6 +422,9 @@ class RefAdapter:
                                          C.c_int, _dp, _dp, _dp]
         L.adapter_render_reference.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_ulonglong, C.c_int,
                                                C.c_int, C.c_int, _dp, _dp, _dp]
+        L.adapter_shared_session.restype = C.c_int
+        L.adapter_shared_session.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int,
+                                             C.c_int] + [_dp] * 7
 
     def scene(self, which) -> AdapterScene:
         return AdapterScene(self.lib, which)
@@ -446,3 +449,19 @@ class RefAdapter:
 
     def render_reference(self, s, res, n, seed_base=0, hero_c=4, depth=8, threads=1):
         return self._render(self.lib.adapter_render_reference, s, res, n, seed_base, hero_c, depth, threads)
+
+    def shared_session(self, s, res, res2, n, batches, k, hero_c=4, depth=8):
+        """gpref::SharedSession over two cameras: (films, hits0, energy) uninterrupted and
+        (films, energy) stopped after k batches and resumed from the downloaded totals;
+        films = [(xyz, hits)] per camera."""
+        a = [np.zeros(res * res * 4), np.zeros(res2 * res2 * 4)]
+        b = [np.zeros(res * res * 4), np.zeros(res2 * res2 * 4)]
+        h0 = np.zeros(res * res)
+        e, re = np.zeros(5), np.zeros(5)
+        ok = self.lib.adapter_shared_session(s.h, res, res2, n, batches, k, hero_c, depth, _ptr(a[0], _dp),
+                                             _ptr(a[1], _dp), _ptr(h0, _dp), _ptr(e, _dp), _ptr(b[0], _dp),
+                                             _ptr(b[1], _dp), _ptr(re, _dp))
+        if not ok:
+            raise RuntimeError("gpref::SharedSession::begin failed")
+        split = lambda v, r: (v[:3 * r * r].reshape(-1, 3), v[3 * r * r:])
+        return ([split(a[0], res), split(a[1], res2)], h0, e, [split(b[0], res), split(b[1], res2)], re)

@@ -17,6 +17,11 @@
 //   adapter_render_gpu                   gpref::renderForward (the HIP library)
 //   adapter_render_reference             the reference's CPU forward path (the loop of
 //                                       renderForward's CPU branch, src/main.cpp:11305-11340)
+//   adapter_shared_session               gpref::SharedSession over two cameras, as the
+//                                       progressive loop drives sharedForwardGpu*
+//                                       (src/main.cpp:19041-19236): uninterrupted, and
+//                                       stopped after k batches then resumed from the
+//                                       downloaded totals as seedFilms / seedEnergy
 // glibc declares `extern int daylight` in <time.h>, which collides with the
 // reference's `Spectrum daylight(double)` (src/lights.h:32); including <ctime> first
 // and renaming the reference's symbol avoids the clash without touching any
@@ -217,6 +222,70 @@ void adapter_render_reference(void* p, int res, long long N, unsigned long long 
     }
     const double v[5] = {e.emitted, e.absorbed, e.sensor, e.escaped, e.residual};
     std::memcpy(e5, v, sizeof v);
+}
+
+// Two cameras (the standard view at res, a second view at res2 x res2), `batches`
+// batches of N photons. out0/out1: the uninterrupted session's running totals of each
+// camera (xyz then hits, 4 doubles per pixel), hits0: its camera-0 poll after the
+// last batch, e: its ledger; rout0/rout1 and re: the same from a session stopped after
+// k batches, downloaded, and a new session seeded with those totals that ran the
+// rest. Returns 1 if every session began, else 0.
+int adapter_shared_session(void* p, int res, int res2, long long N, int batches, int k, int heroC, int maxDepth,
+                           double* out0, double* out1, double* hits0, double* e, double* rout0, double* rout1,
+                           double* re) {
+    auto* h = static_cast<Handle*>(p);
+    gpref::gMaxDepth = maxDepth;
+    Camera c2;
+    c2.lookAt(Vec3{0.2, 0.6, 2.5}, Vec3{0.5, 0.4, 0.5}, Vec3{0, 1, 0}, 45.0, res2, res2);
+    const std::vector<Camera> cams{camera(res), c2};
+    const std::vector<int> rx{res, res2}, ry{res, res2};
+    auto films = [&] {
+        std::vector<Film> f(2);
+        for (int c = 0; c < 2; ++c) { f[c].resX = rx[c]; f[c].resY = ry[c]; f[c].alloc(); }
+        return f;
+    };
+    auto put = [&](const std::vector<Film>& f, double* o0, double* o1) {
+        double* o[2] = {o0, o1};
+        for (int c = 0; c < 2; ++c) {
+            const size_t np = f[c].hits.size();
+            copyFilm(f[c], o[c], o[c] + 3 * np);
+        }
+    };
+    auto putE = [](const EnergyReport& r, double* o) {
+        o[0] = r.emitted; o[1] = r.absorbed; o[2] = r.sensor; o[3] = r.escaped; o[4] = r.residual;
+    };
+    {
+        gpref::SharedSession s;
+        if (!s.begin(h->scene, cams, rx, ry, 'B', true, heroC, false)) return 0;
+        for (int b = 0; b < batches; ++b) s.batch(N, (unsigned long long)(N * b), true);
+        std::vector<double> hh((size_t)res * res, -1.0);
+        s.hits0(hh);
+        std::memcpy(hits0, hh.data(), hh.size() * sizeof(double));
+        std::vector<Film> f = films();
+        EnergyReport r;
+        s.download(f, r);
+        put(f, out0, out1);
+        putE(r, e);
+    }
+    std::vector<Film> seed = films();
+    EnergyReport seedE;
+    {
+        gpref::SharedSession s;
+        if (!s.begin(h->scene, cams, rx, ry, 'B', true, heroC, false)) return 0;
+        for (int b = 0; b < k; ++b) s.batch(N, (unsigned long long)(N * b), true);
+        s.download(seed, seedE);
+    }
+    {
+        gpref::SharedSession s;
+        if (!s.begin(h->scene, cams, rx, ry, 'B', true, heroC, false, &seed, &seedE)) return 0;
+        for (int b = k; b < batches; ++b) s.batch(N, (unsigned long long)(N * b), true);
+        std::vector<Film> f = films();
+        EnergyReport r;
+        s.download(f, r);
+        put(f, rout0, rout1);
+        putE(r, re);
+    }
+    return 1;
 }
 
 }  // extern "C"

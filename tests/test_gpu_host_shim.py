@@ -1,7 +1,8 @@
 """The C++20 entry point (goodphotons_amd/host/gp_forward.hpp) called from a C++
 program: single-shot renderForwardGpu equals a resident session traced in two
-batches bit for bit, an unsupported camera model returns an empty film, and the
-film matches the oracle on the same scene and photons."""
+batches bit for bit, an unsupported camera model returns an empty film, a two-camera
+SharedSession resumed from its downloaded totals equals the uninterrupted one, and
+the film matches the oracle on the same scene and photons."""
 import os
 import subprocess
 
@@ -43,6 +44,12 @@ def test_cpp_shim(tmp_path, orc):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "unsupported_model_empty 1" in r.stdout
+    # two-camera resident session: resumed from downloaded totals (seeded films) equals
+    # uninterrupted bit for bit; camera 0's hit poll equals its downloaded hits; its
+    # hits equal the single-camera render of the same photons
+    assert "shared_session_resume_exact 1" in r.stdout
+    assert "shared_session_hits0_equal 1" in r.stdout
+    assert "shared_session_hits_equal_single 1" in r.stdout
     x1, h1, e1 = _read(out + "_single.bin")
     x2, h2, e2 = _read(out + "_session.bin")
     assert np.array_equal(x1, x2) and np.array_equal(h1, h2)

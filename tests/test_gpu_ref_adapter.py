@@ -34,3 +34,25 @@ def test_adapter_render_matches_reference(gplib, which, hero_c):
     assert ge["emitted"] == pytest.approx(re["emitted"], rel=1e-6)
     assert abs(gh.sum() / rh.sum() - 1) < 2e-3
     assert t["rel_rmse_max"] < 1e-4 and ge["emitted"] == se["emitted"]
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_adapter_shared_session_resumes_exactly(gplib, which):
+    # sharedForwardGpuBegin / Batch / Hits0 / Download / End with the reference's types:
+    # two cameras, 4 batches; stopped after 2, downloaded, and resumed as seedFilms /
+    # seedEnergy, the totals equal the uninterrupted session's bit for bit (the seed
+    # sums round back to the same fixed-point words); the camera-0 poll equals the
+    # downloaded hit counts; camera 0 matches the single-camera render of the same
+    # photons.
+    adp = oracle.RefAdapter()
+    s = adp.scene(which)
+    res, res2, n = 48, 40, 1 << 16
+    print(f"[adapter session] scene {which}", flush=True)
+    films, h0, e, rfilms, re = adp.shared_session(s, res, res2, n, 4, 2, hero_c=4, depth=8)
+    for (x, h), (rx, rh) in zip(films, rfilms):
+        assert h.sum() > 0 and np.array_equal(x, rx) and np.array_equal(h, rh)
+    assert np.array_equal(h0, films[0][1])
+    assert np.allclose(e, re, rtol=1e-12) and e[0] > 0
+    gx, gh, ge = adp.render_gpu(s, res, 4 * n, hero_c=4, depth=8)
+    assert np.array_equal(gh, films[0][1])
+    assert image_metrics(films[0][0], gx, 4 * n, 4 * n)["rel_rmse_max"] < 1e-5
