@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the c1 line beside the c2 headline")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--ray-sort", action="store_true",
+                    help="bin each iteration's paths by origin cell and direction octant (gp_tuning.ray_sort)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     ap.add_argument("--traversal-stats", default="",
@@ -344,6 +346,8 @@ def main():
     sd = wl["make"]()
     N = args.photons or sd.n_photons
     res = wl["res"]
+    if args.ray_sort:
+        lib.set_tuning(ray_sort=1)
     if args.autotune:
         import ctypes as C
         gs0 = gp.GpuScene(sd, lib)
