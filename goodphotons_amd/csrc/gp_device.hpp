@@ -262,16 +262,17 @@ constexpr int kTravThreads = 256;
 // never overflows. With WITH_T an entry is (node, entry distance) in one 64-bit LDS
 // word; without (any-hit) just the node. SPILL = false is the variant for trees that
 // fit in LDS (no spill area allocated, no spill branches).
-template <int SD, bool WITH_T = true, bool SPILL = true>
+template <int SD, bool WITH_T = true, bool SPILL = true, int ST = kTravThreads>
 struct Stack {
+    static constexpr int kStride = ST;   // lanes of the workgroup (LDS column stride)
     int2* e2;             // LDS, this lane's column of SD entries (WITH_T)
     int* e1;              // LDS, this lane's column of SD entries (!WITH_T)
     int2* spill;          // global, spillDepth entries for this thread (SPILL)
     int sp;
     __device__ __forceinline__ void push(int node, float t) {
         if (!SPILL || sp < SD) {
-            if (WITH_T) e2[sp * kTravThreads] = make_int2(node, __float_as_int(t));
-            else e1[sp * kTravThreads] = node;
+            if (WITH_T) e2[sp * ST] = make_int2(node, __float_as_int(t));
+            else e1[sp * ST] = node;
         } else {
             spill[sp - SD] = make_int2(node, __float_as_int(t));
         }
@@ -281,11 +282,11 @@ struct Stack {
         --sp;
         if (!SPILL || sp < SD) {
             if (WITH_T) {
-                const int2 v = e2[sp * kTravThreads];
+                const int2 v = e2[sp * ST];
                 node = v.x;
                 t = __int_as_float(v.y);
             } else {
-                node = e1[sp * kTravThreads];
+                node = e1[sp * ST];
                 t = 0.0f;
             }
         } else {
@@ -392,8 +393,8 @@ struct TravCount<true> {
     __device__ void tris(int c) { tri += (unsigned)c; }
 };
 
-template <int SD, bool SPILL>
-__device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax, int& node) {
+template <int SD, bool SPILL, int ST>
+__device__ __forceinline__ bool popNear(Stack<SD, true, SPILL, ST>& stk, float tmax, int& node) {
     while (stk.sp > 0) {
         float t;
         stk.pop(node, t);
@@ -406,10 +407,10 @@ __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL>& stk, float tmax,
 // and its distance. Front-to-back: the nearest hit slot is visited next and the
 // others pushed farthest first with their entry distances; a popped entry farther
 // than the current hit is skipped. Loop shape: while-while (Aila & Laine 2009).
-template <int SD, bool SPILL, class Cnt>
+template <int SD, bool SPILL, class Cnt, int ST>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
-                                            float tmin, float& tmax, Stack<SD, true, SPILL>& stk, Cnt& cnt) {
+                                            float tmin, float& tmax, Stack<SD, true, SPILL, ST>& stk, Cnt& cnt) {
     int best = -1;
     int node = 0;
     stk.sp = 0;
@@ -421,11 +422,11 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
             const int n = visit4(r, nodes + 8 * node, tmin, tmax, t, l);
             cnt.node();
             if (!SPILL) {
-                stk.e2[stk.sp * kTravThreads] = make_int2(l[3], __float_as_int(t[3]));
+                stk.e2[stk.sp * ST] = make_int2(l[3], __float_as_int(t[3]));
                 stk.sp += n > 3 ? 1 : 0;
-                stk.e2[stk.sp * kTravThreads] = make_int2(l[2], __float_as_int(t[2]));
+                stk.e2[stk.sp * ST] = make_int2(l[2], __float_as_int(t[2]));
                 stk.sp += n > 2 ? 1 : 0;
-                stk.e2[stk.sp * kTravThreads] = make_int2(l[1], __float_as_int(t[1]));
+                stk.e2[stk.sp * ST] = make_int2(l[1], __float_as_int(t[1]));
                 stk.sp += n > 1 ? 1 : 0;
             } else {
                 if (n > 3) stk.push(l[3], t[3]);

@@ -404,18 +404,21 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ extend
+// One wave per workgroup: a workgroup's LDS (its lanes' stacks) is released as soon as
+// its own rays are done, instead of waiting for the slowest of four waves.
+constexpr int kExtendThreads = 64;
 // STATS: also count the traversal statistics (node visits, triangle tests) per job.
 template <int SD, bool SPILL, bool STATS>
-__global__ void __launch_bounds__(kTravThreads) k_extend(Queues q, int cur) {
+__global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
     constexpr int NA = STATS ? 3 : 1;   // segments[, node visits, triangle tests]
-    __shared__ int2 sStk[SD * kTravThreads];
+    __shared__ int2 sStk[SD * kExtendThreads];
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int rb = q.binned ? 2 : cur;
-    Stack<SD, true, SPILL> stk{sStk + threadIdx.x, nullptr,
-                               SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
+    Stack<SD, true, SPILL, kExtendThreads> stk{sStk + threadIdx.x, nullptr,
+                                               SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
     __shared__ unsigned long long sTab[kMaxJobs * NA];
     initJobTable<NA>(sTab, nJobs);
@@ -1207,8 +1210,8 @@ void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s)
     } while (0)
 
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
-#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q, cur)
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
     if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLS);
     else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
 #undef GP_CALLS

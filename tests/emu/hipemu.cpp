@@ -241,6 +241,13 @@ hipError_t hipFreeAsync(void* p, hipStream_t) { free(p); return hipSuccess; }
 hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
 hipError_t hipHostFree(void* p) { free(p); return hipSuccess; }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memmove(d, s, n); return hipSuccess; }
+hipError_t hipMemcpy2D(void* d, size_t dpitch, const void* s, size_t spitch, size_t width, size_t height,
+                       hipMemcpyKind) {
+    if (width > dpitch || width > spitch) return hipErrorInvalidValue;
+    for (size_t r = 0; r < height; ++r)
+        std::memmove((char*)d + r * dpitch, (const char*)s + r * spitch, width);
+    return hipSuccess;
+}
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
     if (gCapturing) { fprintf(stderr, "EMU: memcpy during capture\n"); abort(); }
     return hipMemcpy(d, s, n, k);
