@@ -1210,8 +1210,8 @@ void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s)
     } while (0)
 
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
-#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend * kExtendGridScale * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend * kExtendGridScale * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
     if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLS);
     else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
 #undef GP_CALLS

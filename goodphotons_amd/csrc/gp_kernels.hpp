@@ -46,6 +46,13 @@ struct Queues {
     unsigned segCap, connSegCap, mirrorSegCap;
 };
 
+// The closest-hit kernel's grid holds kExtendGridScale times the threads of the other
+// traversal kernels (its spill area is sized for that many): smaller shares of the queue
+// per wave even out the waves' unequal traversal costs (c2: 203.5 -> 198.8 ms of
+// closest-hit time per 2^28 photons at scale 4, 167.0 with ray binning; scale 8 no
+// further).
+constexpr int kExtendGridScale = 4;
+
 struct LaunchShape {
     int blocksExtend, threadsExtend;
     int blocksShade, threadsShade;

@@ -440,7 +440,7 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     ls.threadsShade = 256;
     ls.blocksExtend = std::max(1, cus * tun.grid_waves_per_cu / 4);
     ls.blocksShade = std::max(1, cus * tun.grid_waves_per_cu / 4);
-    const size_t threads = (size_t)ls.blocksExtend * 256;
+    const size_t threads = (size_t)ls.blocksExtend * 256 * kExtendGridScale;
     const unsigned sc = segmentBound(cap, (unsigned)ls.blocksShade);
     const int camsNeed = std::max(camCap, std::max(1, cams));
     const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
