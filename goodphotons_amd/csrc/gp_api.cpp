@@ -560,7 +560,8 @@ gp_status gp_trace_shared(gp_scene* s, int32_t n_cams, const gp_camera_desc* cam
     if (!E.tunSet) { E.tun = defaultTuning(); E.tunSet = true; }
     ++E.epoch;
     const int lds = E.tun.lds_stack;
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, s->stackNeed, n_cams, s->nMirrors > 0)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, s->stackNeed, n_cams, s->nMirrors > 0, s->nTris)) != GP_OK)
+        return st;
     E.mirrorActive = s->nMirrors > 0;
     // the statistics variants run for this call only (plain launches, no graph); set
     // after ensure(), which may rebuild the queue descriptor
@@ -735,7 +736,7 @@ gp_status gp_set_tuning(const gp_tuning* in) {
         return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
     if ((in->block_extend != 0 && in->block_extend != 256) || (in->block_shade != 0 && in->block_shade != 256))
         return fail(GP_ERR_INVALID, "block_extend and block_shade are fixed at 256 (0 or 256)");
-    if (in->ray_sort != 0 && in->ray_sort != 1) return fail(GP_ERR_INVALID, "ray_sort must be 0 or 1");
+    if (in->ray_sort < 0 || in->ray_sort > 2) return fail(GP_ERR_INVALID, "ray_sort must be 0, 1 or 2");
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);
     if (E.owner) return fail(GP_ERR_STATE, "a server has jobs in flight on this device");

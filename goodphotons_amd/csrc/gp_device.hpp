@@ -34,6 +34,15 @@ __device__ __forceinline__ F3 normalize(F3 a) {
 __device__ __forceinline__ F3 reflectDir(F3 d, F3 n) { return d - n * (2.0f * dot(d, n)); }
 __device__ __forceinline__ F3 load3(const float* p) { return F3{p[0], p[1], p[2]}; }
 __device__ __forceinline__ F3 xyz(float4 v) { return F3{v.x, v.y, v.z}; }
+// The same 16 bytes read as the other vector type.
+__device__ __forceinline__ uint4 bitcast4(float4 v) {
+    return make_uint4((unsigned)__float_as_int(v.x), (unsigned)__float_as_int(v.y), (unsigned)__float_as_int(v.z),
+                      (unsigned)__float_as_int(v.w));
+}
+__device__ __forceinline__ float4 bitcast4f(uint4 v) {
+    return make_float4(__int_as_float((int)v.x), __int_as_float((int)v.y), __int_as_float((int)v.z),
+                       __int_as_float((int)v.w));
+}
 
 // Loads through a global-address-space pointer. Scene tables are reached through
 // pointers stored in the job record, which the compiler cannot place in an address

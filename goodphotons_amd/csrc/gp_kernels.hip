@@ -147,6 +147,26 @@ __device__ __forceinline__ unsigned xcdBlock() {
     return (G % kXcds) ? b : (b % kXcds) * (G / kXcds) + b / kXcds;
 }
 
+// Ray-binning key of a ray: the Morton code of its origin's cell in the job's
+// kBinCellsPerAxis^3 grid over the triangle-centroid box, then its direction octant.
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+
+__device__ __forceinline__ unsigned binCell(float x, float lo, float scale) {
+    // fmaxf returns 0 for a NaN cell coordinate
+    const float f = fminf(fmaxf((x - lo) * scale, 0.0f), (float)(kBinCellsPerAxis - 1));
+    return (unsigned)f;
+}
+
+__device__ __forceinline__ unsigned short binKey(const DevJob& J, F3 o, F3 d) {
+    const unsigned m = spread3(binCell(o.x, J.binLo[0], J.binScale[0])) |
+                       (spread3(binCell(o.y, J.binLo[1], J.binScale[1])) << 1) |
+                       (spread3(binCell(o.z, J.binLo[2], J.binScale[2])) << 2);
+    const unsigned oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return (unsigned short)((m << 3) | oct);
+}
+
 // Camera connection record for a diffuse (or emitter) vertex at p with shading
 // normal n (flat triangles: n is also the oriented geometric normal, so the
 // reference's terminator factor and adjoint correction are exactly 1).
@@ -398,6 +418,7 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
         q.beta[cur][slot] = make_float4(beta[0], beta[1], beta[2], beta[3]);
         unsigned meta = packMeta((unsigned)j, 0u, (unsigned)C) | (dead ? 0x80000000u : 0u);
         q.st[cur][slot] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), (unsigned)idx, meta);
+        if (q.binned) q.key[cur][slot] = dead ? (unsigned short)0 : binKey(J, o, dir);
     }
     flushJobTable(eacc, eTab, nJobs, ea);
     flushJobTable(pacc, pTab, nJobs, pa);
@@ -408,7 +429,8 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
 // its own rays are done, instead of waiting for the slowest of four waves.
 constexpr int kExtendThreads = 64;
 // STATS: also count the traversal statistics (node visits, triangle tests) per job.
-template <int SD, bool SPILL, bool STATS>
+// BIN: read the binned queue (ray binning) instead of queue `cur`.
+template <int SD, bool SPILL, bool STATS, bool BIN>
 __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
     constexpr int NA = STATS ? 3 : 1;   // segments[, node visits, triangle tests]
     __shared__ int2 sStk[SD * kExtendThreads];
@@ -416,7 +438,6 @@ __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int rb = q.binned ? 2 : cur;
     Stack<SD, true, SPILL, kExtendThreads> stk{sStk + threadIdx.x, nullptr,
                                                SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
@@ -425,15 +446,15 @@ __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
     JobAcc<NA> sacc;
     AccAddr sa{q.jobs, kAccSegments};
     for (unsigned li = gtid; li < n; li += stride) {
-        const unsigned i = q.binned ? li : sm.slot(li);
-        const uint4 s4 = q.st[rb][i];
+        const unsigned i = BIN ? li : sm.slot(li);
+        const uint4 s4 = BIN ? bitcast4(q.pb[5 * (size_t)i + 4]) : q.st[cur][i];
         int best = -1;
         float t = INFINITY;
         if (!(s4.w & 0x80000000u)) {
             const int j = (int)(s4.w & 0xffffu);
             const DevJob& J = q.jobs[j];
-            const float4 o = q.ro[rb][i];
-            const float4 d = q.rd[rb][i];
+            const float4 o = BIN ? q.pb[5 * (size_t)i] : q.ro[cur][i];
+            const float4 d = BIN ? q.pb[5 * (size_t)i + 1] : q.rd[cur][i];
             RayPre r = prepareRay(xyz(o), xyz(d));
             float tmax = INFINITY;
             TravCount<STATS> tc;
@@ -452,6 +473,9 @@ __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
 }
 
 // ------------------------------------------------------------------ shade
+// BIN: read the binned queue (ray binning) instead of queue `cur` and store the
+// survivors' bin keys.
+template <bool BIN>
 __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     const int nxt = cur ^ 1;
     Ctrl* ctl = q.ctrl;
@@ -464,11 +488,12 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     JobAcc<3> eacc;   // absorbed, escaped, residual -> ledger entries 1, 3, 4
     AccAddr ea{q.jobs, kAccEnergy};
     auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
-    const int rb = q.binned ? 2 : cur;
     for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
-        const unsigned i = q.binned ? li : sm.slot(li);
-        const uint4 s4 = q.st[rb][i];
+        const unsigned i = BIN ? li : sm.slot(li);
+        const float4* rec = q.pb + 5 * (size_t)i;   // binned record (BIN only)
+        const uint4 s4 = BIN ? bitcast4(rec[4]) : q.st[cur][i];
         bool alive = false;
+        unsigned short bkey = 0;
         // diffuse vertex to connect to the cameras after shading
         int camN = 0;
         bool mirrorsToo = false;
@@ -488,7 +513,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             int nUp = (int)((meta >> 24) & 7u);
             const DevJob& J = q.jobs[j];
             accumulate(eacc, j, eTab);
-            const float4 l4 = q.lam[rb][i], b4 = q.beta[rb][i];
+            const float4 l4 = BIN ? rec[2] : q.lam[cur][i], b4 = BIN ? rec[3] : q.beta[cur][i];
             lam[0] = l4.x; lam[1] = l4.y; lam[2] = l4.z; lam[3] = l4.w;
             beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z; beta[3] = b4.w;
             auto activeSum = [&]() {
@@ -501,7 +526,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             if (tri < 0) {
                 eacc.v[1] += toFixed(activeSum(), J.energyScale);
             } else {
-                const float4 o4 = q.ro[rb][i], d4 = q.rd[rb][i];
+                const float4 o4 = BIN ? rec[0] : q.ro[cur][i], d4 = BIN ? rec[1] : q.rd[cur][i];
                 const F3 d = xyz(d4);
                 const F3 p = xyz(o4) + d * h.x;
                 const float4 tn = J.trin[tri];
@@ -617,6 +642,8 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
                     eacc.v[2] += toFixed(activeSum(), J.energyScale);
                     alive = false;
                 }
+                if constexpr (BIN)
+                    if (alive) bkey = binKey(J, no, nd);
                 meta = packMeta((unsigned)j, (unsigned)(bounce + 1), (unsigned)nUp);
             }
         }
@@ -629,6 +656,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             q.lam[nxt][pi] = make_float4(lam[0], lam[1], lam[2], lam[3]);
             q.beta[nxt][pi] = make_float4(beta[0], beta[1], beta[2], beta[3]);
             q.st[nxt][pi] = make_uint4((unsigned)rng.state, (unsigned)(rng.state >> 32), s4.z, meta);
+            if constexpr (BIN) q.key[nxt][pi] = bkey;
         }
     }
     flushJobTable(eacc, eTab, nJobs, eaddr);
@@ -940,32 +968,11 @@ __global__ void __launch_bounds__(kTravThreads) k_mirror(Queues q) {
 }
 
 // ------------------------------------------------------------------ ray binning
-// Counting sort of queue `cur` into queue 2 by a 9-bit key: the Morton code of the
-// origin's cell in a 4 x 4 x 4 grid over the job's triangle-centroid box, then the
-// direction octant. Column c (workgroup xcdBlock() of C = q.binCols) owns the
+// Counting sort of queue `cur` into the binned queue by the 12-bit keys the emission
+// and shading kernels stored (binKey). Column c (workgroup xcdBlock() of C = q.binCols) owns the
 // contiguous logical range [n c / C, n (c + 1) / C). The order inside a (column,
 // bin) run is that of LDS atomics; nothing downstream depends on it (films, ledgers
 // and counters are integer sums).
-__device__ __forceinline__ unsigned spread3(unsigned v) {
-    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
-}
-
-__device__ __forceinline__ unsigned binCell(float x, float lo, float scale) {
-    // fmaxf returns 0 for a NaN cell coordinate
-    const float f = fminf(fmaxf((x - lo) * scale, 0.0f), (float)(kBinCellsPerAxis - 1));
-    return (unsigned)f;
-}
-
-__device__ __forceinline__ unsigned binKey(const Queues& q, uint4 s4, float4 o, float4 d) {
-    if (s4.w & 0x80000000u) return 0u;
-    const DevJob& J = q.jobs[s4.w & 0xffffu];
-    const unsigned m = spread3(binCell(o.x, J.binLo[0], J.binScale[0])) |
-                       (spread3(binCell(o.y, J.binLo[1], J.binScale[1])) << 1) |
-                       (spread3(binCell(o.z, J.binLo[2], J.binScale[2])) << 2);
-    const unsigned oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
-    return (m << 3) | oct;
-}
-
 __device__ __forceinline__ void binRange(unsigned n, unsigned col, unsigned cols, unsigned& lo, unsigned& hi) {
     lo = (unsigned)(((unsigned long long)n * col) / cols);
     hi = (unsigned)(((unsigned long long)n * (col + 1)) / cols);
@@ -981,10 +988,10 @@ __global__ void __launch_bounds__(256) k_bin_count(Queues q, int cur) {
     const unsigned col = xcdBlock();
     unsigned lo, hi;
     binRange(n, col, (unsigned)q.binCols, lo, hi);
-    for (unsigned li = lo + threadIdx.x; li < hi; li += blockDim.x) {
-        const unsigned i = sm.slot(li);
-        atomicAdd(&h[binKey(q, q.st[cur][i], q.ro[cur][i], q.rd[cur][i])], 1u);
-    }
+    const unsigned short* __restrict__ keys = q.key[cur];
+    // keys are masked: paths queued before binning was switched on carry no key (both
+    // passes read the same words, so any value still gives a permutation)
+    for (unsigned li = lo + threadIdx.x; li < hi; li += blockDim.x) atomicAdd(&h[keys[sm.slot(li)] & (kBins - 1)], 1u);
     __syncthreads();
     unsigned* out = q.binHist + (size_t)col * kBins;
     for (int b = threadIdx.x; b < kBins; b += blockDim.x) out[b] = h[b];
@@ -1062,7 +1069,7 @@ __global__ void __launch_bounds__(kBinScanThreads) k_bin_scan(Queues q) {
     }
 }
 
-// Moves column xcdBlock()'s paths to their bins' slots of queue 2.
+// Moves column xcdBlock()'s paths to their bins' slots of the binned queue.
 __global__ void __launch_bounds__(256) k_bin_scatter(Queues q, int cur) {
     __shared__ unsigned slot[kBins];
     const unsigned col = xcdBlock();
@@ -1076,33 +1083,35 @@ __global__ void __launch_bounds__(256) k_bin_scatter(Queues q, int cur) {
     const unsigned room = kSegs * q.segCap;
     unsigned lo, hi;
     binRange(n, col, (unsigned)q.binCols, lo, hi);
+    const unsigned short* __restrict__ keys = q.key[cur];
     const uint4* __restrict__ sst = q.st[cur];
     const float4* __restrict__ sro = q.ro[cur];
     const float4* __restrict__ srd = q.rd[cur];
     const float4* __restrict__ slam = q.lam[cur];
     const float4* __restrict__ sbeta = q.beta[cur];
-    uint4* __restrict__ dst_st = q.st[2];
-    float4* __restrict__ dro = q.ro[2];
-    float4* __restrict__ drd = q.rd[2];
-    float4* __restrict__ dlam = q.lam[2];
-    float4* __restrict__ dbeta = q.beta[2];
-    // two paths per step: both loaded before either is stored
+    float4* __restrict__ out = q.pb;
+    // two paths per step: both loaded before either is stored; each lands as one
+    // contiguous 80-byte record
     for (unsigned li = lo + threadIdx.x; li < hi; li += 2 * blockDim.x) {
         const unsigned lj = li + blockDim.x;
         const bool two = lj < hi;
         const unsigned i0 = sm.slot(li), i1 = two ? sm.slot(lj) : i0;
+        const unsigned k0 = keys[i0] & (kBins - 1), k1 = keys[i1] & (kBins - 1);
         const uint4 s0 = sst[i0], s1 = sst[i1];
         const float4 o0 = sro[i0], o1 = sro[i1], d0 = srd[i0], d1 = srd[i1];
         const float4 l0 = slam[i0], l1 = slam[i1], b0 = sbeta[i0], b1 = sbeta[i1];
-        const unsigned k0 = binKey(q, s0, o0, d0), k1 = binKey(q, s1, o1, d1);
         const unsigned t0 = atomicAdd(&slot[k0], 1u);
         const unsigned t1 = two ? atomicAdd(&slot[k1], 1u) : 0u;
-        if (t0 >= room || (two && t1 >= room)) {   // impossible: the two passes see the same keys
+        if (t0 >= room || (two && t1 >= room)) {   // impossible: the two passes read the same keys
             q.ctrl->overflow = 1u;
             continue;
         }
-        dro[t0] = o0; drd[t0] = d0; dlam[t0] = l0; dbeta[t0] = b0; dst_st[t0] = s0;
-        if (two) { dro[t1] = o1; drd[t1] = d1; dlam[t1] = l1; dbeta[t1] = b1; dst_st[t1] = s1; }
+        float4* r0 = out + 5 * (size_t)t0;
+        r0[0] = o0; r0[1] = d0; r0[2] = l0; r0[3] = b0; r0[4] = bitcast4f(s0);
+        if (two) {
+            float4* r1 = out + 5 * (size_t)t1;
+            r1[0] = o1; r1[1] = d1; r1[2] = l1; r1[3] = b1; r1[4] = bitcast4f(s1);
+        }
     }
 }
 
@@ -1187,7 +1196,8 @@ void launchEmit(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) 
     hipLaunchKernelGGL(k_emit, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
 }
 void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
+    if (q.binned) hipLaunchKernelGGL(k_shade<true>, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(ls.blocksShade), dim3(256), 0, s, q, cur);
 }
 // Kernel instances by LDS stack depth and spill use.
 #define GP_BY_STACK(LDS, SPILL, CALL) \
@@ -1210,10 +1220,20 @@ void launchShade(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s)
     } while (0)
 
 void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s) {
-#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false>), dim3(ls.blocksExtend * kExtendGridScale * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
-#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true>), dim3(ls.blocksExtend * kExtendGridScale * (kTravThreads / kExtendThreads)), dim3(kExtendThreads), 0, s, q, cur)
-    if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLS);
-    else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+    const dim3 grid(ls.blocksExtend * kExtendGridScale * (kTravThreads / kExtendThreads));
+#define GP_CALL(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false, false>), grid, dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALLS(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true, false>), grid, dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALLB(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, false, true>), grid, dim3(kExtendThreads), 0, s, q, cur)
+#define GP_CALLSB(SD, SP) hipLaunchKernelGGL((k_extend<SD, SP, true, true>), grid, dim3(kExtendThreads), 0, s, q, cur)
+    if (q.binned) {
+        if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLSB);
+        else GP_BY_STACK(q.lds, q.useSpill, GP_CALLB);
+    } else {
+        if (q.stats) GP_BY_STACK(q.lds, q.useSpill, GP_CALLS);
+        else GP_BY_STACK(q.lds, q.useSpill, GP_CALL);
+    }
+#undef GP_CALLSB
+#undef GP_CALLB
 #undef GP_CALLS
 #undef GP_CALL
 }

@@ -10,15 +10,19 @@ namespace gp {
 // iteration k reads queue (k & 1) and writes survivors to the other. Each queue is
 // kSegs segments of segCap (paths) / connSegCap (connections) slots; a logical index
 // maps to (segment, offset) through the prefix of the per-segment counts. With ray
-// binning (binned = 1) a third path queue holds the iteration's paths in bin order,
-// packed from slot 0 (logical index = slot): the closest-hit and shading kernels read
-// it instead of queue (k & 1).
+// binning (binned = 1) the emission and shading kernels also write each path's bin key,
+// and a third path queue holds the iteration's paths in bin order, packed from slot 0
+// (logical index = slot) as 80-byte records (origin, direction, wavelengths, weights,
+// state: five float4): the closest-hit and shading kernels read it instead of queue
+// (k & 1).
 struct Queues {
-    float4* ro[3];
-    float4* rd[3];
-    float4* lam[3];
-    float4* beta[3];
-    uint4* st[3];
+    float4* ro[2];
+    float4* rd[2];
+    float4* lam[2];
+    float4* beta[2];
+    uint4* st[2];
+    unsigned short* key[2];   // bin key of every path of queue 0 / 1 (ray binning only)
+    float4* pb;               // the binned queue, five float4 per path (ray binning only)
     float2* hit;
     float4* co;      // connection origin.xyz, segment end
     float4* cd;      // connection dir.xyz, pixel index
@@ -39,8 +43,8 @@ struct Queues {
     int stats;       // 1: k_extend / k_connect count traversal statistics per job
     int binned;      // 1: ray binning on (the binning kernels run and queue 2 is read)
     int binCols;     // histogram columns (a multiple of kBinColStep, <= kBinCols)
-    unsigned* binHist;    // kBinCols x kBins counts, column-major by workgroup; the scan
-                          // turns them into each (column, bin)'s first slot
+    unsigned* binHist;    // kBinCols x kBins counts, one row per column; the scan turns
+                          // them into offsets inside each (bin, chunk)
     unsigned* binChunk;   // kBins x kBinChunks exclusive prefixes of the column chunks
     unsigned* binTotal;   // kBins totals, then (after the scan) kBins first slots
     unsigned segCap, connSegCap, mirrorSegCap;

@@ -324,7 +324,7 @@ gp_tuning defaultTuning() {
     t.use_graph = 1;
     t.lds_stack = 16;
     t.profile = 0;
-    t.ray_sort = 0;
+    t.ray_sort = 2;   // automatic: binned for scenes of kBinAutoTris triangles and more
     return t;
 }
 
@@ -432,7 +432,11 @@ gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned lo
     return GP_OK;
 }
 
-gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mirrors) {
+bool binningFor(const gp_tuning& t, int64_t maxTris) {
+    return t.ray_sort == 1 || (t.ray_sort == 2 && maxTris >= kBinAutoTris);
+}
+
+gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mirrors, int64_t maxTris) {
     if (!tunSet) { tun = defaultTuning(); tunSet = true; }
     DeviceState& d = device();
     const int cus = d.cus > 0 ? d.cus : 256;
@@ -446,8 +450,8 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     const bool needQueues = (cap != capacity) || sc != segCap || camsNeed != camCap || !dCtrl;
     const bool wantMirror = mirrors || mirrorQueue;
     const bool needMirrorAlloc = wantMirror && (needQueues || !q.mv[0]);
-    const int binned = tun.ray_sort ? 1 : 0;
-    const bool needBinAlloc = binned && (needQueues || !q.ro[2]);
+    const int binned = binningFor(tun, maxTris) ? 1 : 0;
+    const bool needBinAlloc = binned && (needQueues || !q.pb);
     // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
     // spill[sp - SD], and sp < stackNeed). k_connect's stack holds 4-byte entries, so it
     // keeps at least 24 in LDS at no cost in occupancy (its registers already allow
@@ -526,29 +530,27 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
         mirrorQueue = true;
     }
     if (needBinAlloc) {
-        // the binned path queue (same slots as the others) and the counting-sort tables
+        // the binned path queue (same slots as the others, 80-byte records), the keys of
+        // both path queues and the counting-sort tables
         const size_t pcap = (size_t)kSegs * segCap;
-        void* p[8] = {};
-        const size_t bytes[8] = {pcap * sizeof(float4), pcap * sizeof(float4), pcap * sizeof(float4),
-                                 pcap * sizeof(float4), pcap * sizeof(uint4),
-                                 (size_t)kBinCols * kBins * sizeof(unsigned),
+        void* p[6] = {};
+        const size_t bytes[6] = {pcap * 5 * sizeof(float4), pcap * sizeof(unsigned short),
+                                 pcap * sizeof(unsigned short), (size_t)kBinCols * kBins * sizeof(unsigned),
                                  (size_t)kBinChunks * kBins * sizeof(unsigned), (size_t)kBins * sizeof(unsigned)};
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < 6; ++k) {
             hipError_t e = hipMalloc(&p[k], bytes[k]);
             if (e != hipSuccess) {
                 for (int i = 0; i < k; ++i) hipFree(p[i]);
                 return fail(GP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
             }
         }
-        for (int k = 0; k < 8; ++k) bufs[nbufs++] = p[k];
-        q.ro[2] = (float4*)p[0];
-        q.rd[2] = (float4*)p[1];
-        q.lam[2] = (float4*)p[2];
-        q.beta[2] = (float4*)p[3];
-        q.st[2] = (uint4*)p[4];
-        q.binHist = (unsigned*)p[5];
-        q.binChunk = (unsigned*)p[6];
-        q.binTotal = (unsigned*)p[7];
+        for (int k = 0; k < 6; ++k) bufs[nbufs++] = p[k];
+        q.pb = (float4*)p[0];
+        q.key[0] = (unsigned short*)p[1];
+        q.key[1] = (unsigned short*)p[2];
+        q.binHist = (unsigned*)p[3];
+        q.binChunk = (unsigned*)p[4];
+        q.binTotal = (unsigned*)p[5];
     }
     if (needSpillAlloc) {
         const int depthAlloc = std::max(needSpill, spillDepth);

@@ -155,7 +155,8 @@ struct Engine {
 
     // stackNeed: traversal stack entries the traced scenes can need (gp_scene::stackNeed);
     // mirrors: allocate the mirror-vertex queue (kept from then on, like the camera count)
-    gp_status ensure(unsigned cap, int lds, int stackNeed, int cams = 1, bool mirrors = false);
+    // maxTris: the largest triangle count of the traced scenes (automatic ray binning)
+    gp_status ensure(unsigned cap, int lds, int stackNeed, int cams = 1, bool mirrors = false, int64_t maxTris = 0);
     void release();
     void releaseQueues();
     void releaseSpill();
@@ -171,6 +172,9 @@ inline int chunkIterations(const gp_tuning& t) {
     return t.use_graph ? std::max(2, t.iters_per_graph & ~1) : std::max(1, t.iters_per_graph);
 }
 unsigned segmentBound(unsigned cap, unsigned blocks);
+// Whether the wavefront bins its rays (gp_tuning.ray_sort: 1 always, 2 for scenes of at
+// least kBinAutoTris triangles).
+bool binningFor(const gp_tuning& t, int64_t maxTris);
 // Sums the kSegs accumulator rows of one job: ledger[5] (fixed point) and
 // counters[kAccCounters] (the quantities from kAccPhotons on).
 void sumAccRows(const unsigned long long* rows, unsigned long long ledger[5], unsigned long long* counters);

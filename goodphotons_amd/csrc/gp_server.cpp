@@ -397,10 +397,12 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
     gp_status st;
     int maxStackNeed = 0;
     bool mirrors = false;
+    int64_t maxTris = 0;
     for (auto& kv : srv->jobs) {
         if (kv.second.state == JobState::Finished) continue;   // its scene is no longer used
         maxStackNeed = std::max(maxStackNeed, kv.second.desc.scene->stackNeed);
         mirrors = mirrors || kv.second.desc.scene->nMirrors > 0;
+        maxTris = std::max(maxTris, kv.second.desc.scene->nTris);
     }
     const int lds = E.tun.lds_stack;
     if (srv->engineReady && !srv->inflight.empty()) {
@@ -408,14 +410,14 @@ gp_status gp_server_step(gp_server* srv, int32_t maxIterations, int32_t* finishe
         const int need = std::max(0, maxStackNeed - lds);
         const bool connSpill = maxStackNeed > std::max(lds, 24);
         if (need > E.spillDepth || (need > 0) != (E.q.useSpill != 0) || connSpill != (E.q.connUseSpill != 0) ||
-            (mirrors && !E.q.mv[0]) || mirrors != E.mirrorActive) {
+            (mirrors && !E.q.mv[0]) || mirrors != E.mirrorActive || binningFor(E.tun, maxTris) != (E.q.binned != 0)) {
             while (!srv->inflight.empty())
                 if ((st = processOldest(E, srv)) != GP_OK) return st;
             GP_HIP(hipStreamSynchronize(device().stream));
         }
     }
     const unsigned oldCap = E.capacity, oldSeg = E.segCap;
-    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, maxStackNeed, 1, mirrors)) != GP_OK) return st;
+    if ((st = E.ensure((unsigned)E.tun.queue_capacity, lds, maxStackNeed, 1, mirrors, maxTris)) != GP_OK) return st;
     E.mirrorActive = mirrors;
     const bool fresh = !srv->engineReady || E.capacity != oldCap || E.segCap != oldSeg || srv->epoch != E.epoch;
     if (fresh) {
@@ -603,10 +605,12 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
                 t.lds_stack = l;
                 if ((st = trial(t)) != GP_OK) return st;
             }
-    // then ray binning on or off at the best shape
-    {
+    // then ray binning off and on at the best shape (the grid ran with the caller's setting)
+    const int sortFrom = bestT.ray_sort;
+    for (int rs = 0; rs <= 1; ++rs) {
+        if (rs == sortFrom) continue;
         gp_tuning t = bestT;
-        t.ray_sort = bestT.ray_sort ? 0 : 1;
+        t.ray_sort = rs;
         if ((st = trial(t)) != GP_OK) return st;
     }
     if ((st = gp_set_tuning(&bestT)) != GP_OK) return st;

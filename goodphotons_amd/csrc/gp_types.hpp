@@ -45,18 +45,22 @@ constexpr int kAccAnyTriTests = 13;   // camera shadow rays' triangle tests (tra
 constexpr int kAccCounters = 9;       // counters [kAccPhotons, kAccPhotons + kAccCounters)
 
 // Ray binning (gp_tuning.ray_sort): before the closest-hit kernel, the paths of an
-// iteration are reordered by a 9-bit key, the Morton code of the origin's cell in a
-// 4 x 4 x 4 grid over the job's triangle-centroid box followed by the direction
+// iteration are reordered by a 12-bit key, the Morton code of the origin's cell in an
+// 8 x 8 x 8 grid over the job's triangle-centroid box followed by the direction
 // octant, so that the lanes of a wave trace similar rays.
 // A counting sort: binCols workgroups (4 per CU, at most kBinCols) each count their
 // contiguous share of the queue into a histogram column, the columns are scanned in
 // (bin, column) order, and each workgroup scatters its share to the bins' ranges.
-constexpr int kBinBits = 9;
+constexpr int kBinBits = 12;
 constexpr int kBins = 1 << kBinBits;
 constexpr int kBinCols = 1024;        // most histogram columns (workgroups of the count/scatter)
 constexpr int kBinChunks = 8;         // column chunks per bin in the scan
 constexpr int kBinColStep = 8;        // the column count is a multiple of this (and of kBinChunks)
-constexpr int kBinCellsPerAxis = 4;
+// ray_sort = 2 (automatic) bins while a traced scene has at least this many triangles:
+// below it the traversal is too cheap for the sort to pay (c0, 36 triangles: 4.7 ms
+// per 2^22 photons unbinned, 5.2 binned; c1, 8108: 75.7 vs 71.3 ms per 2^26).
+constexpr int64_t kBinAutoTris = 4096;
+constexpr int kBinCellsPerAxis = 8;
 
 // Leaf links are negative: ~((first << 3) | (count - 1)).
 inline constexpr int leafCode(int first, int count) { return ~((first << 3) | (count - 1)); }

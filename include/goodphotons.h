@@ -412,11 +412,12 @@ typedef struct gp_tuning {
     int32_t lds_stack;       /* traversal stack entries per lane held in LDS (8..32)  */
     int32_t profile;         /* 1 = time every kernel launch with HIP events (no graphs) */
     int32_t ray_sort;        /* 1 = reorder each iteration's paths by origin cell
-                                (4 x 4 x 4 over the scene) and direction octant before
+                                (8 x 8 x 8 over the scene) and direction octant before
                                 the closest-hit kernel, a counting sort into a third
                                 path queue, so that the lanes of a wave trace similar
-                                rays; 0 = queue order (default). Results are
-                                identical either way; gp_autotune tries both. */
+                                rays; 0 = queue order; 2 = automatic (default): 1 while
+                                a traced scene has 4096 triangles or more. Results are
+                                identical in every mode; gp_autotune tries 0 and 1. */
 } gp_tuning;
 
 gp_status gp_get_tuning(gp_tuning* out);

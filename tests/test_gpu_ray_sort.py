@@ -1,10 +1,12 @@
-"""Ray binning (gp_tuning.ray_sort): reordering each iteration's paths by origin cell and
-direction octant changes which lanes trace which paths, never what a path computes,
+"""Ray binning (gp_tuning.ray_sort: 0 off, 1 on, 2 automatic): reordering each
+iteration's paths by origin cell and direction octant changes which lanes trace which
+paths, never what a path computes,
 and the film, ledger and counters are integer sums. So every result must be
 bit-identical with and without it: films (fixed-point words), ledgers, segment /
 connection / splat counts and traversal statistics, for single traces (host and
 device trees, flat mirrors, several cameras), small queues that wrap many times, and
-the server's job mix."""
+the server's job mix, and a server whose automatic binning switches on and off while
+paths are in flight."""
 import contextlib
 
 import numpy as np
@@ -137,6 +139,40 @@ def test_binned_server_mix(gplib):
                 srv = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=8, lib=gplib)
                 ids = [srv.submit(gs, sd.camera, n, seed_base=seed, max_depth=depth, rr_depth=rr)
                        for (sd, n, depth, rr, seed), gs in zip(specs, gscenes)]
+                srv.step()
+                res.append([srv.take(j, sd.camera.res_x, sd.camera.res_y) for j, (sd, *_) in zip(ids, specs)])
+                srv.close()
+        for (ra, xa, ha), (rb, xb, hb) in zip(*res):
+            assert np.array_equal(xa, xb) and np.array_equal(ha, hb)
+            assert ra.stats.segments == rb.stats.segments and ra.stats.splats == rb.stats.splats
+    finally:
+        for gs in gscenes:
+            gs.close()
+
+
+def test_automatic_binning_switches_under_a_running_server(gplib):
+    # ray_sort = 2 bins while a live job's scene has 4096 triangles or more: a server
+    # that starts with a small scene (not binned) and then admits a large one switches
+    # binning on with paths of the first job still in the queue (their keys were never
+    # written), and off again when the large job ends while small ones still run (a
+    # 2^14-path queue spreads the jobs over many iterations); every film equals the
+    # unbinned server's.
+    small = (scenes.cornell_c0(48), 1 << 17, 8, 3, 11)
+    large = (scenes.icosphere_c2(4, 32), 1 << 15, 12, 0, 66)
+    tail = (scenes.cornell_c0(32), 1 << 17, 8, 0, 77)
+    assert large[0].n_tris >= 4096 > small[0].n_tris
+    specs = [small, large, tail]
+    gscenes = [gp.GpuScene(sd, gplib) for sd, *_ in specs]
+    try:
+        res = []
+        for sort in (0, 2):
+            with tuning(gplib, ray_sort=sort, queue_capacity=1 << 14):
+                srv = gp.Server(hbm_budget_bytes=1 << 30, max_live_jobs=8, lib=gplib)
+                ids = [srv.submit(gscenes[0], small[0].camera, small[1], seed_base=small[4], max_depth=small[2],
+                                  rr_depth=small[3])]
+                srv.step(max_iterations=4)
+                for (sd, n, depth, rr, seed), gs in zip(specs[1:], gscenes[1:]):
+                    ids.append(srv.submit(gs, sd.camera, n, seed_base=seed, max_depth=depth, rr_depth=rr))
                 srv.step()
                 res.append([srv.take(j, sd.camera.res_x, sd.camera.res_y) for j, (sd, *_) in zip(ids, specs)])
                 srv.close()
