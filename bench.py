@@ -46,7 +46,12 @@ HBM_PEAK_GBPS = 8000.0
 #   connect: read connection (48 B)                                           per connection
 #            + scene reads as extend (V'_n, V'_p of the shadow rays)
 #            film X, Y, Z, hits read-modify-write (4 x 8 B)                   per splat
+#   bin    : (ray binning) read key (2 B) twice, read path (80 B), write the
+#            binned record (80 B)                                             per segment
+#            emit and shade also write each new path's key (2 B)
 EXTEND_B_PER_SEG = 56
+BIN_B_PER_SEG = 164
+KEY_B = 2
 SHADE_B_PER_SEG = 104
 PATH_B = 80
 CONN_B = 48
@@ -55,13 +60,14 @@ NODE_B = 32      # one binary node visit: a box (24 B) and its links (8 B) in fl
 TRI_B = 48       # one leaf test: three float32 vertices + padding
 
 
-def kernel_bytes(st, trav=None):
+def kernel_bytes(st, trav=None, binned=False):
     """Algorithmic bytes of one trace by kernel. `trav` (scene-resident traffic, only
     for a scene larger than the caches): per-segment V_n, V_p and per-connection
-    V'_n, V'_p of the reference's binary tree."""
+    V'_n, V'_p of the reference's binary tree. `binned`: the trace binned its rays."""
     seg, ph, conn, spl = st["segments"], st["photons"], st["connections"], st["splats"]
     direct = st.get("direct_connections", 0)
     survivors = max(0, seg - ph)
+    key = KEY_B if binned else 0
     ext = seg * EXTEND_B_PER_SEG
     con = conn * CONN_B + spl * SPLAT_B
     if trav:
@@ -69,10 +75,11 @@ def kernel_bytes(st, trav=None):
         con += conn * (trav["Va_n"] * NODE_B + trav["Va_p"] * TRI_B)
     return {
         "k_extend": ext,
-        "k_shade": seg * SHADE_B_PER_SEG + survivors * PATH_B + (conn - direct) * CONN_B,
-        "k_emit": ph * PATH_B + direct * CONN_B,
+        "k_shade": seg * SHADE_B_PER_SEG + survivors * (PATH_B + key) + (conn - direct) * CONN_B,
+        "k_emit": ph * (PATH_B + key) + direct * CONN_B,
         "k_connect": con,
         "k_schedule": 0,
+        "k_bin": seg * BIN_B_PER_SEG if binned else 0,
     }
 
 
@@ -316,8 +323,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the c1 line beside the c2 headline")
     ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--ray-sort", action="store_true",
-                    help="bin each iteration's paths by origin cell and direction octant (gp_tuning.ray_sort)")
+    ap.add_argument("--ray-sort", type=int, choices=[0, 1, 2], default=None,
+                    help="gp_tuning.ray_sort: 0 queue order, 1 bin each iteration's paths by origin cell and "
+                         "direction octant, 2 automatic (the library default: binned for scenes of 4096 "
+                         "triangles and more)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     ap.add_argument("--traversal-stats", default="",
@@ -346,8 +355,8 @@ def main():
     sd = wl["make"]()
     N = args.photons or sd.n_photons
     res = wl["res"]
-    if args.ray_sort:
-        lib.set_tuning(ray_sort=1)
+    if args.ray_sort is not None:
+        lib.set_tuning(ray_sort=args.ray_sort)
     if args.autotune:
         import ctypes as C
         gs0 = gp.GpuScene(sd, lib)
@@ -371,7 +380,7 @@ def main():
     tst = step(0, stats=True)
     trav_c = committed_traversal(args.workload)
     trav = trav_c["reference_tree"] if (trav_c and wl["scene_bytes"]) else None
-    kb = kernel_bytes(pst, trav)
+    kb = kernel_bytes(pst, trav, binned=pst["kernel_launches"][5] > 0)
     kms = dict(zip(gp.abi.KERNEL_NAMES, pst["kernel_ms"]))
     kl = dict(zip(gp.abi.KERNEL_NAMES, pst["kernel_launches"]))
     dom = max(kms, key=kms.get)

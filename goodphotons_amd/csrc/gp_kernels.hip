@@ -979,7 +979,7 @@ __device__ __forceinline__ void binRange(unsigned n, unsigned col, unsigned cols
 }
 
 // Histogram of column xcdBlock() into binHist[col][*].
-__global__ void __launch_bounds__(256) k_bin_count(Queues q, int cur) {
+__global__ void __launch_bounds__(kBinThreads) k_bin_count(Queues q, int cur) {
     __shared__ unsigned h[kBins];
     for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = 0u;
     __syncthreads();
@@ -1070,7 +1070,7 @@ __global__ void __launch_bounds__(kBinScanThreads) k_bin_scan(Queues q) {
 }
 
 // Moves column xcdBlock()'s paths to their bins' slots of the binned queue.
-__global__ void __launch_bounds__(256) k_bin_scatter(Queues q, int cur) {
+__global__ void __launch_bounds__(kBinThreads) k_bin_scatter(Queues q, int cur) {
     __shared__ unsigned slot[kBins];
     const unsigned col = xcdBlock();
     const unsigned ch = col / ((unsigned)q.binCols / kBinChunks);
@@ -1238,10 +1238,10 @@ void launchExtend(const Queues& q, int cur, const LaunchShape& ls, hipStream_t s
 #undef GP_CALL
 }
 void launchBin(const Queues& q, int cur, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_count, dim3(q.binCols), dim3(256), 0, s, q, cur);
+    hipLaunchKernelGGL(k_bin_count, dim3(q.binCols), dim3(kBinThreads), 0, s, q, cur);
     hipLaunchKernelGGL(k_bin_chunks, dim3(std::min(kBins / 32, q.binCols)), dim3(kBinChunks * 32), 0, s, q);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kBinScanThreads), 0, s, q);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(q.binCols), dim3(256), 0, s, q, cur);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(q.binCols), dim3(kBinThreads), 0, s, q, cur);
 }
 void launchConnect(const Queues& q, const LaunchShape& ls, hipStream_t s) {
 #define GP_CALL(SD, SP) hipLaunchKernelGGL((k_connect<SD, SP, false>), dim3(ls.blocksExtend), dim3(kTravThreads), 0, s, q)

@@ -56,6 +56,7 @@ constexpr int kBins = 1 << kBinBits;
 constexpr int kBinCols = 1024;        // most histogram columns (workgroups of the count/scatter)
 constexpr int kBinChunks = 8;         // column chunks per bin in the scan
 constexpr int kBinColStep = 8;        // the column count is a multiple of this (and of kBinChunks)
+constexpr int kBinThreads = 1024;     // threads of a count/scatter workgroup (one column)
 // ray_sort = 2 (automatic) bins while a traced scene has at least this many triangles:
 // below it the traversal is too cheap for the sort to pay (c0, 36 triangles: 4.7 ms
 // per 2^22 photons unbinned, 5.2 binned; c1, 8108: 75.7 vs 71.3 ms per 2^26).
