@@ -327,6 +327,8 @@ def main():
                     help="gp_tuning.ray_sort: 0 queue order, 1 bin each iteration's paths by origin cell and "
                          "direction octant, 2 automatic (the library default: binned for scenes of 4096 "
                          "triangles and more)")
+    ap.add_argument("--lds-stack", type=int, choices=[8, 16, 24, 32], default=None,
+                    help="gp_tuning.lds_stack: LDS traversal stack entries per lane (default 8)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     ap.add_argument("--traversal-stats", default="",
@@ -357,6 +359,8 @@ def main():
     res = wl["res"]
     if args.ray_sort is not None:
         lib.set_tuning(ray_sort=args.ray_sort)
+    if args.lds_stack is not None:
+        lib.set_tuning(lds_stack=args.lds_stack)
     if args.autotune:
         import ctypes as C
         gs0 = gp.GpuScene(sd, lib)

@@ -194,7 +194,9 @@ __device__ __forceinline__ F3 glossyDir(F3 mdir, float rough, Rng& rng) {
 // ------------------------------------------------------------------ intersection
 struct RayPre {
     F3 o, d, inv;     // inv = 1 / d (infinite for a zero component)
-    F3 invF, cl, ch;  // closest-hit slab test: a plane at x is crossed at fma(x, invF, cl | ch)
+    F3 invF, cn, cf;  // closest-hit slab test: the near plane of an axis is crossed at
+                      // fma(x, invF, cn), the far plane at fma(x, invF, cf)
+    int nx, ny, nz;   // float4 index of the near-plane bounds of axis x, y, z in a node
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
@@ -216,16 +218,20 @@ __device__ __forceinline__ RayPre prepareRay(F3 o, F3 d) {
     r.inv = F3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     auto invOf = [](float c) { return fabsf(c) < 1e-30f ? copysignf(1e30f, c) : 1.0f / c; };
     r.invF = F3{invOf(d.x), invOf(d.y), invOf(d.z)};
-    auto addends = [](float oc, float inv, float& cl, float& ch) {
+    auto addends = [](float oc, float inv, float& cn, float& cf) {
         const float p = oc * inv;
         const float e = fabsf(p) * 2.38418579e-7f;   // 2^-22 |o inv|
-        const float s = inv < 0.0f ? -e : e;          // lo is the near plane when inv > 0
-        cl = -p - s;
-        ch = -p + s;
+        cn = -p - e;
+        cf = -p + e;
     };
-    addends(o.x, r.invF.x, r.cl.x, r.ch.x);
-    addends(o.y, r.invF.y, r.cl.y, r.ch.y);
-    addends(o.z, r.invF.z, r.cl.z, r.ch.z);
+    addends(o.x, r.invF.x, r.cn.x, r.cf.x);
+    addends(o.y, r.invF.y, r.cn.y, r.cf.y);
+    addends(o.z, r.invF.z, r.cn.z, r.cf.z);
+    // a node holds lo then hi of each axis: the near plane is lo for a positive
+    // direction component, hi for a negative one (the inverse's sign, -0 included)
+    r.nx = r.invF.x < 0.0f ? 1 : 0;
+    r.ny = r.invF.y < 0.0f ? 3 : 2;
+    r.nz = r.invF.z < 0.0f ? 5 : 4;
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay && ax >= az) ? 0 : ((ay >= az) ? 1 : 2);
     int kx = kz + 1;
@@ -333,51 +339,53 @@ struct Stack {
 
 // Triangles of one leaf: all loads are issued before the first test, so a leaf of k
 // triangles waits for memory once, not k times.
-struct LeafTris {
-    float4 v[kLeafMax][3];
-};
-__device__ __forceinline__ void loadLeaf(const float4* __restrict__ tris, int first, int count, LeafTris& L) {
-#pragma unroll
-    for (int k = 0; k < kLeafMax; ++k) {
-        if (k < count) {
-            const float4* tp = tris + 3 * (first + k);
-            L.v[k][0] = ldg4(tp);
-            L.v[k][1] = ldg4(tp + 1);
-            L.v[k][2] = ldg4(tp + 2);
-        }
-    }
-}
-
 // Slab test of the four slot boxes of a four-wide node: entry distances, hit flags and
 // links. Exit distances are widened by 2^-21 relative so rounding can only over-report
-// a box (Ize 2013, robust BVH traversal). FMA: one fused multiply-add per plane with
-// the ray's conservative addends (closest hit: -2% instructions' time on c2); without,
-// (x - o) * inv (any hit: the addends' six registers cost k_connect a wave per SIMD).
+// a box (Ize 2013, robust BVH traversal). FMA (closest hit): the ray loads each axis's
+// near-plane bounds and far-plane bounds by its direction signs (RayPre::nx..nz), so
+// entry = max of the near planes' fused multiply-adds and exit = min of the far planes'
+// with no per-axis min/max (the same distances as min/max of the two planes: with
+// lo <= hi and cn <= cf the near plane's FMA is never the larger); -24 instructions per
+// node visit. Without FMA (any hit): (x - o) * inv with the exact inverse and per-axis
+// min/max, which also orders the planes of a zero direction component (the addends'
+// six registers would cost k_connect a wave per SIMD).
 template <bool FMA>
 __device__ __forceinline__ void slab4(const RayPre& r, const float4* __restrict__ nd, float tmin, float tmax,
                                       float tin[4], bool h[4], int l[4]) {
     const float widen = 1.00000048f;
-    const float4 lx = ldg4(nd), hx = ldg4(nd + 1), ly = ldg4(nd + 2), hy = ldg4(nd + 3);
-    const float4 lz = ldg4(nd + 4), hz = ldg4(nd + 5), lk = ldg4(nd + 6);
-    const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
-    l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
+    if constexpr (FMA) {
+        const float4 nx = ldg4(nd + r.nx), fx = ldg4(nd + (r.nx ^ 1));
+        const float4 ny = ldg4(nd + r.ny), fy = ldg4(nd + (r.ny ^ 1));
+        const float4 nz = ldg4(nd + r.nz), fz = ldg4(nd + (r.nz ^ 1)), lk = ldg4(nd + 6);
+        const float NX[4] = {nx.x, nx.y, nx.z, nx.w}, FX[4] = {fx.x, fx.y, fx.z, fx.w};
+        const float NY[4] = {ny.x, ny.y, ny.z, ny.w}, FY[4] = {fy.x, fy.y, fy.z, fy.w};
+        const float NZ[4] = {nz.x, nz.y, nz.z, nz.w}, FZ[4] = {fz.x, fz.y, fz.z, fz.w};
+        l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        float x0, x1, y0, y1, z0, z1;
-        if constexpr (FMA) {
-            x0 = fmaf(LX[k], r.invF.x, r.cl.x); x1 = fmaf(HX[k], r.invF.x, r.ch.x);
-            y0 = fmaf(LY[k], r.invF.y, r.cl.y); y1 = fmaf(HY[k], r.invF.y, r.ch.y);
-            z0 = fmaf(LZ[k], r.invF.z, r.cl.z); z1 = fmaf(HZ[k], r.invF.z, r.ch.z);
-        } else {
-            x0 = (LX[k] - r.o.x) * r.inv.x; x1 = (HX[k] - r.o.x) * r.inv.x;
-            y0 = (LY[k] - r.o.y) * r.inv.y; y1 = (HY[k] - r.o.y) * r.inv.y;
-            z0 = (LZ[k] - r.o.z) * r.inv.z; z1 = (HZ[k] - r.o.z) * r.inv.z;
+        for (int k = 0; k < 4; ++k) {
+            const float xn = fmaf(NX[k], r.invF.x, r.cn.x), xf = fmaf(FX[k], r.invF.x, r.cf.x);
+            const float yn = fmaf(NY[k], r.invF.y, r.cn.y), yf = fmaf(FY[k], r.invF.y, r.cf.y);
+            const float zn = fmaf(NZ[k], r.invF.z, r.cn.z), zf = fmaf(FZ[k], r.invF.z, r.cf.z);
+            tin[k] = fmaxf(fmaxf(xn, yn), fmaxf(zn, tmin));
+            const float tout = fminf(fminf(xf, yf), fminf(zf, tmax)) * widen;
+            h[k] = tin[k] <= tout;
         }
-        tin[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        const float tout = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * widen;
-        h[k] = tin[k] <= tout;
+    } else {
+        const float4 lx = ldg4(nd), hx = ldg4(nd + 1), ly = ldg4(nd + 2), hy = ldg4(nd + 3);
+        const float4 lz = ldg4(nd + 4), hz = ldg4(nd + 5), lk = ldg4(nd + 6);
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        l[0] = __float_as_int(lk.x); l[1] = __float_as_int(lk.y); l[2] = __float_as_int(lk.z); l[3] = __float_as_int(lk.w);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x0 = (LX[k] - r.o.x) * r.inv.x, x1 = (HX[k] - r.o.x) * r.inv.x;
+            const float y0 = (LY[k] - r.o.y) * r.inv.y, y1 = (HY[k] - r.o.y) * r.inv.y;
+            const float z0 = (LZ[k] - r.o.z) * r.inv.z, z1 = (HZ[k] - r.o.z) * r.inv.z;
+            tin[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            const float tout = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * widen;
+            h[k] = tin[k] <= tout;
+        }
     }
 }
 
@@ -484,13 +492,14 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
         if (!live) break;
         const int code = ~node;
         const int first = code >> 3, count = (code & 7) + 1;
-        LeafTris L;
-        loadLeaf(tris, first, count, L);
         cnt.tris(count);
-#pragma unroll
-        for (int k = 0; k < kLeafMax; ++k) {
+        // one triangle at a time: a whole-leaf prefetch holds 48 more registers, which
+        // costs the closest-hit kernel a resident wave per SIMD (c2: 4% slower)
+        for (int k = 0; k < count; ++k) {
+            const float4* tp = tris + 3 * (first + k);
+            const float4 v0 = ldg4(tp), v1 = ldg4(tp + 1), v2 = ldg4(tp + 2);
             float t;
-            if (k < count && hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t)) {
+            if (hitTri(r, xyz(v0), xyz(v1), xyz(v2), tmin, tmax, t)) {
                 tmax = t;
                 best = first + k;
             }
@@ -544,16 +553,13 @@ __device__ __forceinline__ bool traceAny(const float4* __restrict__ nodes,
         if (!live) break;
         const int code = ~node;
         const int first = code >> 3, count = (code & 7) + 1;
-        LeafTris L;
-        loadLeaf(tris, first, count, L);
         bool hit = false;
-#pragma unroll
-        for (int k = 0; k < kLeafMax; ++k) {
+        for (int k = 0; k < count && !hit; ++k) {
+            const float4* tp = tris + 3 * (first + k);
+            const float4 v0 = ldg4(tp), v1 = ldg4(tp + 1), v2 = ldg4(tp + 2);
             float t;
-            if (k < count && !hit) {
-                cnt.tris(1);
-                hit = hitTri(r, xyz(L.v[k][0]), xyz(L.v[k][1]), xyz(L.v[k][2]), tmin, tmax, t);
-            }
+            cnt.tris(1);
+            hit = hitTri(r, xyz(v0), xyz(v1), xyz(v2), tmin, tmax, t);
         }
         if (hit) return true;
         if (stk.sp == 0) break;

@@ -430,9 +430,17 @@ __global__ void __launch_bounds__(256) k_emit(Queues q, int cur) {
 constexpr int kExtendThreads = 64;
 // STATS: also count the traversal statistics (node visits, triangle tests) per job.
 // BIN: read the binned queue (ray binning) instead of queue `cur`.
+// The traversal waits on dependent node loads, so resident waves are what hides the
+// latency. Segments are counted by k_shade (same paths), so outside the statistics
+// variant the kernel's LDS is the traversal stack alone (SD * 512 bytes per one-wave
+// workgroup), and the launch bound asks for six waves per SIMD (at most 80 registers;
+// the compiler applies it where the LDS allows six: SD = 8, two spilled registers in
+// the spilling binned variant). Measured on c2, closest-hit time per 2^28 photons:
+// 148.3 ms at SD 16 and five waves, 144.2 at SD 8 and five, 140.6 at SD 8 and six;
+// seven or eight waves spill inside the loop (187 / 202 ms).
 template <int SD, bool SPILL, bool STATS, bool BIN>
-__global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
-    constexpr int NA = STATS ? 3 : 1;   // segments[, node visits, triangle tests]
+__global__ void __launch_bounds__(kExtendThreads, 6) k_extend(Queues q, int cur) {
+    constexpr int NA = STATS ? 2 : 1;   // node visits, triangle tests
     __shared__ int2 sStk[SD * kExtendThreads];
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
@@ -441,10 +449,10 @@ __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
     Stack<SD, true, SPILL, kExtendThreads> stk{sStk + threadIdx.x, nullptr,
                                                SPILL ? q.spill + (size_t)gtid * q.spillDepth : nullptr, 0};
     const int nJobs = q.ctrl->nJobs;
-    __shared__ unsigned long long sTab[kMaxJobs * NA];
-    initJobTable<NA>(sTab, nJobs);
+    __shared__ unsigned long long sTab[STATS ? kMaxJobs * NA : 1];
+    if constexpr (STATS) initJobTable<NA>(sTab, nJobs);
     JobAcc<NA> sacc;
-    AccAddr sa{q.jobs, kAccSegments};
+    AccAddr sa{q.jobs, kAccNodeVisits};
     for (unsigned li = gtid; li < n; li += stride) {
         const unsigned i = BIN ? li : sm.slot(li);
         const uint4 s4 = BIN ? bitcast4(q.pb[5 * (size_t)i + 4]) : q.st[cur][i];
@@ -460,16 +468,15 @@ __global__ void __launch_bounds__(kExtendThreads) k_extend(Queues q, int cur) {
             TravCount<STATS> tc;
             best = traceClosest<SD, SPILL>(J.nodes, J.tris, r, kRayEps, tmax, stk, tc);
             t = tmax;
-            accumulate(sacc, j, sTab);
-            sacc.v[0] += 1;
             if constexpr (STATS) {
-                sacc.v[1] += tc.nodes;
-                sacc.v[2] += tc.tri;
+                accumulate(sacc, j, sTab);
+                sacc.v[0] += tc.nodes;
+                sacc.v[1] += tc.tri;
             }
         }
         q.hit[i] = make_float2(t, __int_as_float(best));
     }
-    flushJobTable(sacc, sTab, nJobs, sa);
+    if constexpr (STATS) flushJobTable(sacc, sTab, nJobs, sa);
 }
 
 // ------------------------------------------------------------------ shade
@@ -483,11 +490,15 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
     const unsigned n = loadSegMap(ctl->pathCount[cur], q.segCap, sm);
     const unsigned stride = gridDim.x * blockDim.x;
     const int nJobs = ctl->nJobs;
-    __shared__ unsigned long long eTab[kMaxJobs * 3];
-    initJobTable<3>(eTab, nJobs);
-    JobAcc<3> eacc;   // absorbed, escaped, residual -> ledger entries 1, 3, 4
+    __shared__ unsigned long long eTab[kMaxJobs * 4];
+    initJobTable<4>(eTab, nJobs);
+    // absorbed, escaped, residual -> ledger entries 1, 3, 4; the closest-hit queries of
+    // the paths (every live path of the queue was traced by k_extend) -> segments
+    JobAcc<4> eacc;
     AccAddr ea{q.jobs, kAccEnergy};
-    auto eaddr = [&](int j, int k) { return ea(j, k == 0 ? 1 : (k == 1 ? 3 : 4)); };
+    auto eaddr = [&](int j, int k) {
+        return ea(j, k == 0 ? 1 : (k == 1 ? 3 : (k == 2 ? 4 : kAccSegments - kAccEnergy)));
+    };
     for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
         const unsigned i = BIN ? li : sm.slot(li);
         const float4* rec = q.pb + 5 * (size_t)i;   // binned record (BIN only)
@@ -513,6 +524,7 @@ __global__ void __launch_bounds__(256) k_shade(Queues q, int cur) {
             int nUp = (int)((meta >> 24) & 7u);
             const DevJob& J = q.jobs[j];
             accumulate(eacc, j, eTab);
+            eacc.v[3] += 1;
             const float4 l4 = BIN ? rec[2] : q.lam[cur][i], b4 = BIN ? rec[3] : q.beta[cur][i];
             lam[0] = l4.x; lam[1] = l4.y; lam[2] = l4.z; lam[3] = l4.w;
             beta[0] = b4.x; beta[1] = b4.y; beta[2] = b4.z; beta[3] = b4.w;

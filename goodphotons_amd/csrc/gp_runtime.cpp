@@ -322,7 +322,11 @@ gp_tuning defaultTuning() {
     t.grid_waves_per_cu = 64;
     t.iters_per_graph = 8;
     t.use_graph = 1;
-    t.lds_stack = 16;
+    // 8 LDS stack entries (deeper entries spill to global memory): the closest-hit
+    // kernel then holds 4 KB of LDS per one-wave workgroup and six waves per SIMD
+    // (c2: 140.6 ms of closest-hit time per 2^28 photons against 148.3 with 16 entries
+    // and five waves); k_connect keeps at least 16.
+    t.lds_stack = 8;
     t.profile = 0;
     t.ray_sort = 2;   // automatic: binned for scenes of kBinAutoTris triangles and more
     return t;
@@ -454,10 +458,11 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     const bool needBinAlloc = binned && (needQueues || !q.pb);
     // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
     // spill[sp - SD], and sp < stackNeed). k_connect's stack holds 4-byte entries, so it
-    // keeps at least 24 in LDS at no cost in occupancy (its registers already allow
-    // only four waves per SIMD); the spill area sized for the other kernels covers it.
+    // keeps at least 16 in LDS: 31 KB per 256-lane workgroup, five waves per SIMD as its
+    // registers allow (24 entries would leave room for four: +6% k_connect time on c2);
+    // the spill area sized for the other kernels covers it.
     const int needSpill = std::max(0, stackNeed - lds);
-    const int connLds = std::max(lds, 24);
+    const int connLds = std::max(lds, 16);
     const int connUseSpill = stackNeed > connLds ? 1 : 0;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run
