@@ -457,12 +457,11 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     const int binned = binningFor(tun, maxTris) ? 1 : 0;
     const bool needBinAlloc = binned && (needQueues || !q.pb);
     // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
-    // spill[sp - SD], and sp < stackNeed). k_connect's stack holds 4-byte entries, so it
-    // keeps at least 16 in LDS: 31 KB per 256-lane workgroup, five waves per SIMD as its
-    // registers allow (24 entries would leave room for four: +6% k_connect time on c2);
-    // the spill area sized for the other kernels covers it.
+    // spill[sp - SD], and sp < stackNeed). k_connect keeps as many entries in LDS as the
+    // other kernels (4-byte entries: at 8, six waves per SIMD fit its LDS; 16 leave room
+    // for five, 24 for four); the spill area sized for the other kernels covers it.
     const int needSpill = std::max(0, stackNeed - lds);
-    const int connLds = std::max(lds, 16);
+    const int connLds = lds;
     const int connUseSpill = stackNeed > connLds ? 1 : 0;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run

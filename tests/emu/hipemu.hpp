@@ -61,6 +61,10 @@ __attribute__((noinline)) inline unsigned __shfl(unsigned v, int src, int width 
 __attribute__((noinline)) inline unsigned __shfl_xor(unsigned v, int m, int width = 64) { (void)width; return (unsigned)emu_shfl_i((int)v, -1 - m, __builtin_return_address(0)); }
 __attribute__((noinline)) inline int __shfl_xor(int v, int m, int width = 64) { (void)width; return emu_shfl_i(v, -1 - m, __builtin_return_address(0)); }
 __attribute__((noinline)) inline void __syncthreads() { emu_sync(__builtin_return_address(0)); }
+// A wave barrier orders the lanes of one wave (on the GPU they run in lockstep and the
+// builtin only fences code motion): emulated as a wave collective.
+__attribute__((noinline)) inline void emu_wave_barrier() { (void)emu_ballot(1, __builtin_return_address(0)); }
+#define __builtin_amdgcn_wave_barrier() emu_wave_barrier()
 __attribute__((noinline)) inline int __syncthreads_or(int p) { return emu_sync_or(p, __builtin_return_address(0)); }
 inline unsigned __lane_id() { return emu_lane(); }
 inline unsigned emu_mbcnt_lo(unsigned m, unsigned acc) {
