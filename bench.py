@@ -327,8 +327,8 @@ def main():
                     help="gp_tuning.ray_sort: 0 queue order, 1 bin each iteration's paths by origin cell and "
                          "direction octant, 2 automatic (the library default: binned for scenes of 4096 "
                          "triangles and more)")
-    ap.add_argument("--lds-stack", type=int, choices=[8, 16, 24, 32], default=None,
-                    help="gp_tuning.lds_stack: LDS traversal stack entries per lane (default 8)")
+    ap.add_argument("--lds-stack", type=int, choices=[8, 12, 16, 24, 32], default=None,
+                    help="gp_tuning.lds_stack: LDS traversal stack entries per lane (default 12)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     ap.add_argument("--traversal-stats", default="",

@@ -732,8 +732,9 @@ gp_status gp_set_tuning(const gp_tuning* in) {
     if (in->grid_waves_per_cu < 4 || in->grid_waves_per_cu > 64) return fail(GP_ERR_INVALID, "grid_waves_per_cu must be 4..64");
     if (in->iters_per_graph < 1 || in->iters_per_graph > 64 || (in->use_graph && in->iters_per_graph < 2))
         return fail(GP_ERR_INVALID, "iters_per_graph must be 2..64 (1..64 without graphs)");
-    if (in->lds_stack != 8 && in->lds_stack != 16 && in->lds_stack != 24 && in->lds_stack != 32)
-        return fail(GP_ERR_INVALID, "lds_stack must be 8, 16, 24 or 32");
+    if (in->lds_stack != 8 && in->lds_stack != 12 && in->lds_stack != 16 && in->lds_stack != 24 &&
+        in->lds_stack != 32)
+        return fail(GP_ERR_INVALID, "lds_stack must be 8, 12, 16, 24 or 32");
     if ((in->block_extend != 0 && in->block_extend != 256) || (in->block_shade != 0 && in->block_shade != 256))
         return fail(GP_ERR_INVALID, "block_extend and block_shade are fixed at 256 (0 or 256)");
     if (in->ray_sort < 0 || in->ray_sort > 2) return fail(GP_ERR_INVALID, "ray_sort must be 0, 1 or 2");

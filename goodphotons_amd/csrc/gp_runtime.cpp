@@ -322,11 +322,11 @@ gp_tuning defaultTuning() {
     t.grid_waves_per_cu = 64;
     t.iters_per_graph = 8;
     t.use_graph = 1;
-    // 8 LDS stack entries (deeper entries spill to global memory): the closest-hit
-    // kernel then holds 4 KB of LDS per one-wave workgroup and six waves per SIMD
-    // (c2: 140.6 ms of closest-hit time per 2^28 photons against 148.3 with 16 entries
-    // and five waves); k_connect keeps at least 16.
-    t.lds_stack = 8;
+    // 12 LDS stack entries (deeper entries spill to global memory): the closest-hit
+    // kernel then holds 6 KB of LDS per one-wave workgroup and six waves per SIMD
+    // (c2: 139.0 ms of closest-hit time per 2^28 photons, 140.9 with 8 entries, 148.3
+    // with 16 entries and five waves); k_connect keeps 8.
+    t.lds_stack = 12;
     t.profile = 0;
     t.ray_sort = 2;   // automatic: binned for scenes of kBinAutoTris triangles and more
     return t;
@@ -458,10 +458,11 @@ gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mi
     const bool needBinAlloc = binned && (needQueues || !q.pb);
     // Entries beyond the LDS stack live in the spill area (stack index sp >= SD maps to
     // spill[sp - SD], and sp < stackNeed). k_connect keeps as many entries in LDS as the
-    // other kernels (4-byte entries: at 8, six waves per SIMD fit its LDS; 16 leave room
-    // for five, 24 for four); the spill area sized for the other kernels covers it.
-    const int needSpill = std::max(0, stackNeed - lds);
-    const int connLds = lds;
+    // closest-hit kernel, 8 for its 12 (4-byte entries: at 8, six waves per SIMD fit its
+    // LDS; 12 or 16 leave room for five, 24 for four); k_mirror runs its 16-entry
+    // variant for 12. The spill area covers the shallowest of them.
+    const int connLds = lds == 12 ? 8 : lds;
+    const int needSpill = std::max(0, stackNeed - std::min(lds, connLds));
     const int connUseSpill = stackNeed > connLds ? 1 : 0;
     const bool needSpillAlloc = needSpill > spillDepth || (needSpill > 0 && threads > spillThreads);
     // The kernels without spill code keep the whole stack in LDS: they may only run

@@ -570,7 +570,7 @@ gp_status gp_autotune(gp_scene* scene, const gp_camera_desc* cam, const gp_trace
     if (st != GP_OK) return st;
     const int caps[] = {1 << 22, 1 << 23, 1 << 24};
     const int waves[] = {32, 64};
-    const int ldss[] = {8, 16};
+    const int ldss[] = {8, 12, 16};
     double bestRate = -1.0;
     gp_tuning bestT = base;
     // a failed trial leaves the caller's tuning in place

@@ -409,9 +409,9 @@ typedef struct gp_tuning {
     int32_t iters_per_graph; /* iterations per host read of the wavefront state (per
                                 captured hipGraph: even, 2..64; 1..64 without graphs) */
     int32_t use_graph;       /* 1 = replay captured graphs, 0 = plain launches       */
-    int32_t lds_stack;       /* traversal stack entries per lane held in LDS: 8, 16,
-                                24 or 32 (default 8; deeper entries spill to global
-                                memory)                                             */
+    int32_t lds_stack;       /* traversal stack entries per lane held in LDS: 8, 12,
+                                16, 24 or 32 (default 12; deeper entries spill to
+                                global memory)                                       */
     int32_t profile;         /* 1 = time every kernel launch with HIP events (no graphs) */
     int32_t ray_sort;        /* 1 = reorder each iteration's paths by origin cell
                                 (8 x 8 x 8 over the scene) and direction octant before

@@ -87,7 +87,7 @@ def test_invalid_arguments_rejected_before_device():
     t = abi.Tuning(queue_capacity=1000, grid_waves_per_cu=32, iters_per_graph=8, lds_stack=16)
     assert lib.gp_set_tuning(C.byref(t)) == 1
     t.queue_capacity = 1 << 16
-    t.lds_stack = 12
+    t.lds_stack = 20
     assert lib.gp_set_tuning(C.byref(t)) == 1
 
 
