@@ -257,6 +257,8 @@ __device__ __forceinline__ float edgeFallback(float a, float b, float c, float d
 // Watertight two-sided ray/triangle test (Woop, Benthin, Wald 2013) in float.
 // The three edge-function differences are evaluated without FMA contraction so a
 // shared edge is decided identically (with opposite sign) by both triangles.
+// CLOSED: accept t == tmax too (the closest-hit tie rule decides such hits).
+template <bool CLOSED = false>
 __device__ __forceinline__ bool hitTri(const RayPre& r, F3 v0, F3 v1, F3 v2, float tmin, float tmax,
                                        float& tOut) {
 #pragma clang fp contract(off)
@@ -283,7 +285,7 @@ __device__ __forceinline__ bool hitTri(const RayPre& r, F3 v0, F3 v1, F3 v2, flo
     if (tmin > 0.0f && !((T > 0.0f) == (det > 0.0f) && T != 0.0f)) return false;
     if (fabsf(T) > tmax * fabsf(det) * 1.0000010f) return false;
     float t = T * (1.0f / det);
-    if (!(t >= tmin && t < tmax)) return false;
+    if (!(t >= tmin && (CLOSED ? t <= tmax : t < tmax))) return false;
     tOut = t;
     return true;
 }
@@ -455,16 +457,22 @@ __device__ __forceinline__ bool popNear(Stack<SD, true, SPILL, ST>& stk, float t
 // Closest hit over the four-wide tree. Returns the leaf-order triangle index (or -1)
 // and its distance. Front-to-back: the nearest hit slot is visited next and the
 // others pushed farthest first with their entry distances; a popped entry farther
-// than the current hit is skipped. Loop shape: while-while (Aila & Laine 2009).
+// than the current hit is skipped. Loop shape: while-while with postponed leaves
+// (Aila & Laine 2009, speculative traversal): a lane that reaches a leaf keeps it and
+// walks on through inner nodes until every lane of the wave still in the node loop
+// holds one, so that the lanes visit nodes together and test leaves together.
+// The visit order therefore depends on the wave's other rays; the result does not:
+// among triangles at exactly the closest distance the lowest index wins (each of them
+// is tested, since no box on its way is entered beyond that distance).
 template <int SD, bool SPILL, class Cnt, int ST>
 __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const RayPre& r,
                                             float tmin, float& tmax, Stack<SD, true, SPILL, ST>& stk, Cnt& cnt) {
+    constexpr int kDone = -2147483647 - 1;   // no node left (leaf links are ~code, code < 2^31 - 1)
     int best = -1;
-    int node = 0;
+    int node = 0, leaf = 0;          // leaf: a postponed leaf link, 0 = none
     stk.sp = 0;
     while (true) {
-        bool live = true;
         while (node >= 0) {
             float t[4];
             int l[4];
@@ -482,29 +490,36 @@ __device__ __forceinline__ int traceClosest(const float4* __restrict__ nodes,
                 if (n > 2) stk.push(l[2], t[2]);
                 if (n > 1) stk.push(l[1], t[1]);
             }
-            if (n > 0) {
-                node = l[0];
-            } else if (!popNear(stk, tmax, node)) {
-                live = false;
-                break;
+            if (n > 0) node = l[0];
+            else if (!popNear(stk, tmax, node)) node = kDone;
+            if (node < 0 && node != kDone && leaf == 0) {
+                leaf = node;
+                if (!popNear(stk, tmax, node)) node = kDone;
+            }
+            if (__ballot(leaf == 0) == 0ull) break;
+        }
+        while (leaf != 0) {
+            const int code = ~leaf;
+            const int first = code >> 3, count = (code & 7) + 1;
+            cnt.tris(count);
+            // one triangle at a time: a whole-leaf prefetch holds 48 more registers,
+            // which costs the closest-hit kernel a resident wave per SIMD (c2: 4% slower)
+            for (int k = 0; k < count; ++k) {
+                const float4* tp = tris + 3 * (first + k);
+                const float4 v0 = ldg4(tp), v1 = ldg4(tp + 1), v2 = ldg4(tp + 2);
+                float t;
+                if (hitTri<true>(r, xyz(v0), xyz(v1), xyz(v2), tmin, tmax, t) && (t < tmax || first + k < best)) {
+                    tmax = t;
+                    best = first + k;
+                }
+            }
+            leaf = 0;
+            if (node < 0 && node != kDone) {
+                leaf = node;
+                if (!popNear(stk, tmax, node)) node = kDone;
             }
         }
-        if (!live) break;
-        const int code = ~node;
-        const int first = code >> 3, count = (code & 7) + 1;
-        cnt.tris(count);
-        // one triangle at a time: a whole-leaf prefetch holds 48 more registers, which
-        // costs the closest-hit kernel a resident wave per SIMD (c2: 4% slower)
-        for (int k = 0; k < count; ++k) {
-            const float4* tp = tris + 3 * (first + k);
-            const float4 v0 = ldg4(tp), v1 = ldg4(tp + 1), v2 = ldg4(tp + 2);
-            float t;
-            if (hitTri(r, xyz(v0), xyz(v1), xyz(v2), tmin, tmax, t)) {
-                tmax = t;
-                best = first + k;
-            }
-        }
-        if (!popNear(stk, tmax, node)) break;
+        if (node == kDone) break;
     }
     return best;
 }

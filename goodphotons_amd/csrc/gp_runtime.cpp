@@ -328,7 +328,7 @@ gp_tuning defaultTuning() {
     // with 16 entries and five waves); k_connect keeps 8.
     t.lds_stack = 12;
     t.profile = 0;
-    t.ray_sort = 2;   // automatic: binned for scenes of kBinAutoTris triangles and more
+    t.ray_sort = 2;   // automatic: binned for scenes of kBinAutoTris to kBinAutoMaxTris triangles
     return t;
 }
 
@@ -437,7 +437,7 @@ gp_status Engine::readJobAcc(int slot, unsigned long long ledger[5], unsigned lo
 }
 
 bool binningFor(const gp_tuning& t, int64_t maxTris) {
-    return t.ray_sort == 1 || (t.ray_sort == 2 && maxTris >= kBinAutoTris);
+    return t.ray_sort == 1 || (t.ray_sort == 2 && maxTris >= kBinAutoTris && maxTris < kBinAutoMaxTris);
 }
 
 gp_status Engine::ensure(unsigned cap, int lds, int stackNeed, int cams, bool mirrors, int64_t maxTris) {

@@ -172,8 +172,8 @@ inline int chunkIterations(const gp_tuning& t) {
     return t.use_graph ? std::max(2, t.iters_per_graph & ~1) : std::max(1, t.iters_per_graph);
 }
 unsigned segmentBound(unsigned cap, unsigned blocks);
-// Whether the wavefront bins its rays (gp_tuning.ray_sort: 1 always, 2 for scenes of at
-// least kBinAutoTris triangles).
+// Whether the wavefront bins its rays (gp_tuning.ray_sort: 1 always, 2 when the largest
+// traced scene has kBinAutoTris to kBinAutoMaxTris - 1 triangles).
 bool binningFor(const gp_tuning& t, int64_t maxTris);
 // Sums the kSegs accumulator rows of one job: ledger[5] (fixed point) and
 // counters[kAccCounters] (the quantities from kAccPhotons on).

@@ -57,10 +57,16 @@ constexpr int kBinCols = 1024;        // most histogram columns (workgroups of t
 constexpr int kBinChunks = 8;         // column chunks per bin in the scan
 constexpr int kBinColStep = 8;        // the column count is a multiple of this (and of kBinChunks)
 constexpr int kBinThreads = 1024;     // threads of a count/scatter workgroup (one column)
-// ray_sort = 2 (automatic) bins while a traced scene has at least this many triangles:
-// below it the traversal is too cheap for the sort to pay (c0, 36 triangles: 4.7 ms
-// per 2^22 photons unbinned, 5.2 binned; c1, 8108: 75.7 vs 71.3 ms per 2^26).
+// ray_sort = 2 (automatic) bins while the largest traced scene has at least
+// kBinAutoTris and fewer than kBinAutoMaxTris triangles. Below, the traversal is too
+// cheap for the sort to pay (c0, 36 triangles: 3.75 G segments/s unbinned, 3.46 G
+// binned); at a million triangles and more the closest-hit kernel's postponed-leaf
+// traversal gains less from the sort than the sort costs (1M / 2M random soups 0.84 vs
+// 0.81 G, the 1.3M-triangle icosphere 2.57 vs 2.39 G, c2's 5.24M 2.51 vs 2.47 G); c1
+// (8,064 triangles) gains: 3.10 vs 2.98 G. Between 8K and 1.3M triangles the bound is
+// not measured.
 constexpr int64_t kBinAutoTris = 4096;
+constexpr int64_t kBinAutoMaxTris = 1 << 16;
 constexpr int kBinCellsPerAxis = 8;
 
 // Leaf links are negative: ~((first << 3) | (count - 1)).

@@ -214,7 +214,9 @@ typedef struct gp_trace_stats {
      * slab-tested (a leaf is a link of its parent, not a node) and triangles
      * intersected by the segments' closest hits, and the same for the camera shadow
      * rays' any hits (up to the first occluder). Flat-mirror connection legs are not
-     * counted. */
+     * counted. The closest-hit counts depend slightly on which rays share a wave (a
+     * lane keeps visiting nodes while its wave-mates look for a leaf); the any-hit
+     * counts are a function of the rays alone. */
     uint64_t node_visits;
     uint64_t tri_tests;
     uint64_t shadow_node_visits;
@@ -418,8 +420,9 @@ typedef struct gp_tuning {
                                 the closest-hit kernel, a counting sort into a third
                                 path queue, so that the lanes of a wave trace similar
                                 rays; 0 = queue order; 2 = automatic (default): 1 while
-                                a traced scene has 4096 triangles or more. Results are
-                                identical in every mode; gp_autotune tries 0 and 1. */
+                                the largest traced scene has 4096 to 65535 triangles.
+                                Results are identical in every mode; gp_autotune tries
+                                0 and 1. */
 } gp_tuning;
 
 gp_status gp_get_tuning(gp_tuning* out);

@@ -2,8 +2,11 @@
 TraversalStats / -bvhstats, src/bvh.h:79, src/main.cpp:2599), on the device.
 
 The counting kernel variants must leave the film and the path statistics unchanged bit
-for bit; the counts are a function of the traced rays only, so they repeat exactly run
-to run and across queue capacities; every query visits at least the root node. Against
+for bit; every query visits at least the root node. The shadow rays' any-hit counts are
+a function of the traced rays only, so they repeat exactly across queue capacities; the
+closest-hit counts also depend on which rays share a wave (a lane that reached a leaf
+keeps visiting nodes while its wave-mates look for one), so with another queue capacity
+they repeat within a few percent. Against
 the oracle's counts over the reference's binary tree on the same photons: a four-wide
 node visit replaces up to three binary inner-node visits and a leaf costs no visit of
 its own, so the four-wide tree visits fewer nodes; both trees stop at leaves of at most
@@ -47,14 +50,18 @@ def test_traversal_stats(gplib, orc, name, make, n, depth):
     assert all(s0[k] == 0 for k in keys)
     assert s1["node_visits"] >= s1["segments"] and s1["shadow_node_visits"] >= s1["connections"]
     assert 0 < s1["direct_connections"] <= s1["photons"] < s1["connections"]
-    # deterministic: the same counts with another queue capacity (other scheduling)
+    # another queue capacity (other scheduling, other wave-mates): the any-hit counts
+    # repeat exactly, the closest-hit counts within 5%
     old = gplib.tuning()
     try:
         gplib.set_tuning(queue_capacity=1 << 15)
         _, s2 = _run(gs, sd, n, depth, True)
     finally:
         gplib.set_tuning(**{k: getattr(old, k) for k, _ in old._fields_})
-    assert all(s2[k] == s1[k] for k in keys)
+    print(f"[trav] {name} at 2^15 slots: node_visits={s2['node_visits']} tri_tests={s2['tri_tests']}", flush=True)
+    assert s2["shadow_node_visits"] == s1["shadow_node_visits"] and s2["shadow_tri_tests"] == s1["shadow_tri_tests"]
+    for k in ("node_visits", "tri_tests"):
+        assert abs(s2[k] - s1[k]) <= 0.05 * s1[k], k
     # against the reference's binary tree on the same photons (the oracle's counters)
     _, _, _, c = orc.render(orc.scene(sd), sd.camera, n, hero_c=4, max_bounce=depth, rr_depth=sd.rr_depth,
                             threads=8)
