@@ -109,7 +109,25 @@ struct SegMap {
             if (i >= pre[g]) { base = (unsigned)g * cap; off = i - pre[g]; }
         return base + off;
     }
+    // The same slot through a per-workgroup LDS table tab[g] = g * cap - pre[g]
+    // (initSegTab): the segment is counted with compares against the wave-uniform
+    // prefixes and the one per-lane lookup is an LDS read, so no uniform value has to
+    // be copied into vector registers (slot's selects need them there).
+    __device__ __forceinline__ unsigned slotTab(unsigned i, const unsigned* tab) const {
+        unsigned g = 0;
+#pragma unroll
+        for (int k = 1; k < kSegs; ++k) g += (i >= pre[k]) ? 1u : 0u;
+        return i + tab[g];
+    }
 };
+
+// Fills the LDS table of SegMap::slotTab; the caller synchronises the workgroup before
+// the first lookup.
+__device__ __forceinline__ void initSegTab(const SegMap& m, unsigned* tab) {
+#pragma unroll
+    for (int g = 0; g < kSegs; ++g)
+        if (threadIdx.x == (unsigned)g) tab[g] = (unsigned)g * m.cap - m.pre[g];
+}
 
 __device__ __forceinline__ unsigned loadSegMap(const unsigned (*cnt)[kLine], unsigned cap, SegMap& m) {
     unsigned acc = 0;
@@ -443,8 +461,13 @@ template <int SD, bool SPILL, bool STATS, bool BIN>
 __global__ void __launch_bounds__(kExtendThreads, 6) k_extend(Queues q, int cur) {
     constexpr int NA = STATS ? 2 : 1;   // node visits, triangle tests
     __shared__ int2 sStk[SD * kExtendThreads];
+    __shared__ unsigned segTab[kSegs];
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->pathCount[cur], q.segCap, sm);
+    if constexpr (!BIN) {
+        initSegTab(sm, segTab);
+        __syncthreads();
+    }
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD, true, SPILL, kExtendThreads> stk{sStk + threadIdx.x, nullptr,
@@ -455,7 +478,7 @@ __global__ void __launch_bounds__(kExtendThreads, 6) k_extend(Queues q, int cur)
     JobAcc<NA> sacc;
     AccAddr sa{q.jobs, kAccNodeVisits};
     for (unsigned li = gtid; li < n; li += stride) {
-        const unsigned i = BIN ? li : sm.slot(li);
+        const unsigned i = BIN ? li : sm.slotTab(li, segTab);
         const uint4 s4 = BIN ? bitcast4(q.pb[5 * (size_t)i + 4]) : q.st[cur][i];
         int best = -1;
         float t = INFINITY;
@@ -482,15 +505,16 @@ __global__ void __launch_bounds__(kExtendThreads, 6) k_extend(Queues q, int cur)
 
 // ------------------------------------------------------------------ shade
 // BIN: read the binned queue (ray binning) instead of queue `cur` and store the
-// survivors' bin keys. The binned variant asks for five waves per SIMD (96 registers,
-// no spills; -3% k_shade time on c2); the queue-order variant would spill at 96 and
-// keeps the compiler's choice.
+// survivors' bin keys. Launch bound: five waves per SIMD (96 registers, no spills;
+// -3% k_shade time on c2 binned).
 template <bool BIN>
-__global__ void __launch_bounds__(256, BIN ? 5 : 1) k_shade(Queues q, int cur) {
+__global__ void __launch_bounds__(256, 5) k_shade(Queues q, int cur) {
     const int nxt = cur ^ 1;
     Ctrl* ctl = q.ctrl;
     SegMap sm;
     const unsigned n = loadSegMap(ctl->pathCount[cur], q.segCap, sm);
+    __shared__ unsigned segTab[kSegs];
+    if constexpr (!BIN) initSegTab(sm, segTab);   // initJobTable's barrier covers it
     const unsigned stride = gridDim.x * blockDim.x;
     const int nJobs = ctl->nJobs;
     __shared__ unsigned long long eTab[kMaxJobs * 4];
@@ -503,7 +527,7 @@ __global__ void __launch_bounds__(256, BIN ? 5 : 1) k_shade(Queues q, int cur) {
         return ea(j, k == 0 ? 1 : (k == 1 ? 3 : (k == 2 ? 4 : kAccSegments - kAccEnergy)));
     };
     for (unsigned li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += stride) {
-        const unsigned i = BIN ? li : sm.slot(li);
+        const unsigned i = BIN ? li : sm.slotTab(li, segTab);
         const float4* rec = q.pb + 5 * (size_t)i;   // binned record (BIN only)
         const uint4 s4 = BIN ? bitcast4(rec[4]) : q.st[cur][i];
         bool alive = false;
@@ -754,6 +778,8 @@ __global__ void __launch_bounds__(kTravThreads, 6) k_connect(Queues q) {
     clearSplats(sKey, sVal);
     SegMap sm;
     const unsigned n = loadSegMap(q.ctrl->connCount, q.connSegCap, sm);
+    __shared__ unsigned segTab[kSegs];
+    initSegTab(sm, segTab);   // initJobTable's barrier covers it
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack<SD, false, SPILL> stk{nullptr, sStk + threadIdx.x,
@@ -769,7 +795,7 @@ __global__ void __launch_bounds__(kTravThreads, 6) k_connect(Queues q) {
         bool pending = false;
         unsigned long long key = 0, X = 0, Y = 0, Z = 0, H = 0;
         if (li < n) {
-            const unsigned i = sm.slot(li);
+            const unsigned i = sm.slotTab(li, segTab);
             const float4 co = q.co[i], cd = q.cd[i], cx = q.cx[i];
             const unsigned tag = (unsigned)__float_as_int(cx.w);
             const int j = (int)(tag & 0xffffu);
