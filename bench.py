@@ -329,6 +329,8 @@ def main():
                          "triangles and more)")
     ap.add_argument("--lds-stack", type=int, choices=[8, 12, 16, 24, 32], default=None,
                     help="gp_tuning.lds_stack: LDS traversal stack entries per lane (default 12)")
+    ap.add_argument("--queue-capacity", type=int, default=None,
+                    help="gp_tuning.queue_capacity: path slots of the wavefront (default 2^23)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (rocprof scaling)")
     ap.add_argument("--traversal-stats", default="",
@@ -361,6 +363,8 @@ def main():
         lib.set_tuning(ray_sort=args.ray_sort)
     if args.lds_stack is not None:
         lib.set_tuning(lds_stack=args.lds_stack)
+    if args.queue_capacity is not None:
+        lib.set_tuning(queue_capacity=args.queue_capacity)
     if args.autotune:
         import ctypes as C
         gs0 = gp.GpuScene(sd, lib)
