@@ -247,3 +247,42 @@ def test_empty_and_tiny_traces(gplib):
     assert st.photons == 7
     with pytest.raises(gp.GpError):           # beyond the film's photon budget
         gp.trace(gs, sd.camera, f, 1, seed_base=8, max_depth=8)
+
+
+def test_exact_ties_resolve_independently_of_wave_mates(gplib):
+    """Every triangle of a soup twice, at identical vertices: every hit is an exact tie.
+    The closest-hit traversal postpones leaves until its wave holds one in every lane,
+    so a ray's visit order depends on its wave-mates; the answer must not. The same rays
+    probed in another order and in a smaller batch return the same primitive and
+    distance, and that is the soup's own closest hit (the single-copy scene's primitive,
+    modulo the copy, at the same distance: the triangle test does not depend on the
+    tree)."""
+    import dataclasses
+    base = scenes.soup(4000, 11, res=32)
+    n = base.n_tris
+    sd = dataclasses.replace(base, verts=np.concatenate([base.verts, base.verts]),
+                             mat_ids=np.concatenate([base.mat_ids, base.mat_ids]))
+    gs = gp.GpuScene(sd, gplib)
+    try:
+        rays = seeded_rays(sd, 40000, 99)
+        p0, t0 = gs.closest_hit(rays, tmin=1e-6)
+        perm = np.random.default_rng(5).permutation(len(rays))
+        p1, t1 = gs.closest_hit(np.ascontiguousarray(rays[perm]), tmin=1e-6)
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(len(perm))
+        p1, t1 = p1[inv], t1[inv]
+        p2, t2 = gs.closest_hit(np.ascontiguousarray(rays[:3001]), tmin=1e-6)
+        hit = p0 >= 0
+        print(f"[ties] rays={len(rays)} hits={int(hit.sum())}", flush=True)
+        assert hit.sum() > 1000
+        assert np.array_equal(p0, p1) and np.array_equal(t0, t1)
+        assert np.array_equal(p0[:3001], p2) and np.array_equal(t0[:3001], t2)
+    finally:
+        gs.close()
+    gb = gp.GpuScene(base, gplib)
+    try:
+        pb, tb = gb.closest_hit(rays, tmin=1e-6)
+    finally:
+        gb.close()
+    assert np.array_equal(pb >= 0, hit)
+    assert np.array_equal(p0[hit] % n, pb[hit]) and np.array_equal(t0[hit], tb[hit])
