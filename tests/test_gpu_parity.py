@@ -190,6 +190,41 @@ def test_queue_capacity_does_not_change_result(gplib):
         gplib.set_tuning(**{k: getattr(old, k) for k, _ in old._fields_})
 
 
+@pytest.mark.parametrize("name,make,n,depth", [("c1", lambda: scenes.caustics_c1(32), 1 << 16, 16),
+                                               ("mirror", lambda: scenes.mirror_box(32), 1 << 15, 8),
+                                               ("c2_l5", lambda: scenes.icosphere_c2(5, 32), 1 << 16, 12)],
+                         ids=["c1", "mirror", "c2_l5"])
+def test_lds_stack_depths_give_identical_films(gplib, name, make, n, depth):
+    """Every LDS stack depth (8, 12, 16, 24, 32 entries; the closest-hit, any-hit, mirror
+    and probe kernels each have their own variants and spill beyond them) traces the
+    same paths: the stack only holds the pending nodes, so films, counters and probe
+    prim ids are bit-identical, in queue order and binned."""
+    sd = make()
+    gs = gp.GpuScene(sd, gplib)
+    old = gplib.tuning()
+    rays = seeded_rays(sd, 4096, 3)
+    try:
+        ref = None
+        for sort in (0, 1):
+            for lds in (8, 12, 16, 24, 32):
+                gplib.set_tuning(lds_stack=lds, ray_sort=sort)
+                f = gp.Film(32, 32, max_photons=n, lib=gplib)
+                _, st = gp.trace(gs, sd.camera, f, n, hero_c=4, max_depth=depth)
+                p, t = gs.closest_hit(rays, tmin=1e-6)
+                got = (f.raw()[0].copy(), st.segments, st.connections, st.splats, p, t)
+                f.close()
+                if ref is None:
+                    ref = got
+                    assert got[1] > n
+                else:
+                    assert np.array_equal(got[0], ref[0]), (sort, lds)
+                    assert got[1:4] == ref[1:4], (sort, lds)
+                    assert np.array_equal(got[4], ref[4]) and np.array_equal(got[5], ref[5]), (sort, lds)
+    finally:
+        gplib.set_tuning(**{k: getattr(old, k) for k, _ in old._fields_})
+        gs.close()
+
+
 @pytest.mark.parametrize("mtype,rough", [(3, 0.0), (abi.MAT_GLOSSY, 0.0), (7, 0.0)],
                          ids=["halfmirror", "glossy_rough0", "grating"])
 def test_unsupported_material_rejected(gplib, mtype, rough):
