@@ -339,8 +339,6 @@ struct Stack {
     }
 };
 
-// Triangles of one leaf: all loads are issued before the first test, so a leaf of k
-// triangles waits for memory once, not k times.
 // Slab test of the four slot boxes of a four-wide node: entry distances, hit flags and
 // links. Exit distances are widened by 2^-21 relative so rounding can only over-report
 // a box (Ize 2013, robust BVH traversal). FMA (closest hit): the ray loads each axis's
