@@ -6,9 +6,9 @@
   mean luminance within 0.3%, segment and connection counts within 2e-3.
 - The 5,242,880-triangle noise-displaced icosphere: closest-hit prim ids at full scale
   against the oracle's BVH (mismatches only on near-ties, plus bounded boundary grazes,
-  by the oracle's brute-force ambiguity report), a 64^2 film and the bench's 1024^2 film
-  at 2^26 photons against the oracle on the same photons, and the full budget's
-  run-to-run and batch-additivity identities.
+  by the oracle's brute-force ambiguity report), a 64^2 film and the bench's own
+  configuration (1024^2, 2^28 photons) against the oracle on the same photons (per-pixel
+  relative RMSE < 1%), and the full budget's run-to-run and batch-additivity identities.
 - The caustics workload at full size: run-to-run bit identity, additivity of
   seed-offset batches, ledger closure in expectation."""
 import numpy as np
@@ -84,13 +84,15 @@ def test_c2_film_matches_oracle(c2, orc):
 
 
 def test_c2_bench_resolution_film_matches_oracle(c2, orc):
-    # The bench's 1024^2 film at a quarter of its photons (2^26, 64 per pixel), against
-    # the oracle on the same photons on all host threads: the paths that part between
-    # float32 and double are per-pixel noise at 64 photons per pixel, so the bar is on
-    # 8x8-pixel boxes (the reference's own comparison, tools/pfmcmp.py:40-63), the mean
-    # luminance and the counts; the per-pixel figure is printed.
+    # The bench's own configuration (1024^2, 2^28 photons, depth 12), against the oracle
+    # on the same photons on all host threads (about a minute on 16): the paths that part
+    # between float32 and double are per-pixel noise that shrinks with photons per pixel
+    # (at 2^26 the worst channel is 1.6%, at 2^28 0.88%), so at the full budget the
+    # per-pixel, per-channel relative RMSE is under 1% (the metric's matched RMSE), and
+    # so are the 8x8-pixel boxes (the reference's own comparison, tools/pfmcmp.py:40-63),
+    # the mean luminance and the counts.
     sd, gs, cs = c2
-    n = 1 << 26
+    n = 1 << 28
     cam = sd.with_camera_res(1024).camera
     gx, gh, ge, gst = gp.render_forward(gs, cam, n, hero_c=4, max_depth=12)
     T = host_threads()
@@ -98,7 +100,7 @@ def test_c2_bench_resolution_film_matches_oracle(c2, orc):
     m = image_metrics(gx, cx, n, n, res=(1024, 1024))
     print(f"[c2 1024^2] film {m} segments gpu {gst['segments']} cpu {cst['segments']} connections gpu "
           f"{gst['connections']} cpu {cst['connections']} (oracle on {T} threads)", flush=True)
-    assert m["rel_rms_box"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
+    assert m["rel_rmse_max"] < 0.01 and m["rel_rms_box"] < 0.01 and abs(m["mean_lum_ratio"] - 1) < 0.003
     assert abs(ge["emitted"] / ce["emitted"] - 1) < 1e-6
     assert abs(gst["segments"] / cst["segments"] - 1) < 2e-3
     assert abs(gst["connections"] / cst["connections"] - 1) < 2e-3
